@@ -1,0 +1,141 @@
+/*
+ * libsddc -- C ABI of the MI355X (gfx950) pseudospectral double-diffusive
+ * convection time step.
+ *
+ * All compute entry points take device pointers and a HIP stream (passed as
+ * void*; NULL = the default stream) and are stream-ordered.  They return an
+ * int status (SDDC_OK = 0); sddc_last_error() gives the message of the last
+ * failure on the calling thread.  No entry point allocates device memory
+ * except sddc_plan_create / sddc_plan_reserve and the first call of a
+ * transform at a new length (its twiddle tables are cached).
+ *
+ * Batches: a plan describes one size class (N_fm, N_r, d, dt, Pr, Tau); B
+ * instances of that class are stored back to back, each in the reference's
+ * state layout X[f*N + k*nr + i] (f = psi,T,S; N = N_fm*nr; nr = N_r-1).
+ * Ra and Ra_s are per-instance device arrays of length B.
+ *
+ * Reference interfaces are cited as file:line of
+ * mannixp/SpectralDoubleDiffusiveConvection.
+ */
+#ifndef SDDC_H
+#define SDDC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum sddc_status {
+  SDDC_OK = 0,
+  SDDC_ERR_INVALID = 1,      /* bad argument (maps to ValueError)          */
+  SDDC_ERR_HIP = 2,          /* HIP runtime error                          */
+  SDDC_ERR_UNSUPPORTED = 3,  /* size or mode this build does not support   */
+  SDDC_ERR_ODD_NFM = 4       /* odd N_fm (Matrix_Operators.py:758-759)     */
+};
+
+enum sddc_precision {
+  SDDC_FP64 = 0,   /* plain method: everything in float64                  */
+  SDDC_MIXED = 1   /* float32 transforms, products and Chebyshev GEMMs of  */
+                   /* the nonlinear term; float64 linear terms and solves   */
+};
+
+enum sddc_transform_kind { SDDC_IDCT = 0, SDDC_IDST = 1, SDDC_DCT = 2, SDDC_DST = 3 };
+
+/* Host-side operator set, built exactly as Build_Matrix_Operators
+ * (Main.py:179-225) builds it.  Matrices are row-major float64. */
+typedef struct sddc_operators {
+  int N_fm;                 /* theta modes K (even)                          */
+  int N_r;                  /* Chebyshev degree; nr = N_r - 1 unknowns        */
+  double d, dt, Pr, Tau;
+  const double* D;          /* (N_r+1)^2  cheb_radial D (Matrix_Operators.py:10-28) */
+  const double* R;          /* N_r+1      radial points                       */
+  const double* Dr;         /* nr^2  D[1:-1,1:-1] (Matrix_Operators.py:764)   */
+  const double* D2s;        /* nr^2  (D@D)[1:-1,1:-1] (A2_SINE, :217)         */
+  const double* D2r2;       /* nr^2  (diag(1/R^2) D@D)[1:-1,1:-1] (A2_SINE_R2, :498) */
+  const double* IR2v;       /* nr    1/r^2 (A2_SINE, :216)                    */
+  const double* IR4v;       /* nr    1/r^4 (A2_SINE_R2, :496)                 */
+  const double* DT0;        /* nr    A_T/r^2 (Main.py:201-202)                */
+  const double* Rsq;        /* nr    r^2 (R2, Matrix_Operators.py:79-94)      */
+  const double* GR;         /* nr    R_1^2/r^2 (kGR_RT, :113)                 */
+  const double* D2_A4;      /* nr^2  args_A4 D2 (Main.py:212)                 */
+  const double* IR2_A4;     /* nr    diagonal of args_A4 IR2 (Main.py:213)    */
+  const double* IR4_A4;     /* nr    diagonal of args_A4 IR4 (Main.py:214)    */
+  const double* Linv_A4;    /* N_fm*nr^2  A4_TSTEP_MATS(Pr*dt) (Matrix_Operators.py:1089-1112) */
+  const double* Linv_T;     /* N_fm*nr^2  NAB2_TSTEP_MATS(dt)  (Matrix_Operators.py:1014-1030) */
+  const double* Linv_S;     /* N_fm*nr^2  NAB2_TSTEP_MATS(Tau*dt)                              */
+} sddc_operators;
+
+typedef struct sddc_plan sddc_plan;
+
+/* Plan lifecycle.  Replaces the operator tuple of Build_Matrix_Operators
+ * (Main.py:179-225): uploads the operators once and sizes workspace for
+ * max_batch instances.  symmetric=1 applies the equatorial mask of Eq_SYM
+ * (Main.py:137-176) to inputs and outputs. */
+int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric,
+                     int max_batch, sddc_plan** out);
+int sddc_plan_reserve(sddc_plan* plan, int max_batch);
+int sddc_plan_destroy(sddc_plan* plan);
+int sddc_plan_info(const sddc_plan* plan, long long* workspace_bytes, int* lds_nonlinear);
+
+/* Transforms.py:73-129.  rows independent rows of length len_in.
+ *   SDDC_IDCT  IDCT(f_hat, n)  (Transforms.py:116-129)  out rows have n_out
+ *   SDDC_IDST  IDST(g_hat, n)  (Transforms.py:87-100)   (n_out >= len_in)
+ *   SDDC_DCT   DCT(f, n)       (Transforms.py:102-114)  out rows have
+ *   SDDC_DST   DST(g, n)       (Transforms.py:73-85)    min(n_out, len_in) */
+int sddc_transform(int kind, const double* in, double* out, int rows, int len_in,
+                   int n_out, void* stream);
+
+/* theta operators of psi (Matrix_Operators.py); out gets [B][4][N]:
+ *   J_theta_RT (:436-472), DT0_theta (:131-189), A2_SINE (:192-245),
+ *   A2_SINE_R2 (:475-526).  psi is the first N values of each instance. */
+int sddc_theta_ops(sddc_plan* plan, int B, const double* X, double* out, void* stream);
+
+/* Derivatives (Matrix_Operators.py:630-740): the nine (nr, N_fm) slabs
+ * JT,kDpsi,komega,DT,DC,omega,Dpsi,kT,kC as [B][9][nr][N_fm] float64. */
+int sddc_derivatives(sddc_plan* plan, int B, const double* X, double* out, void* stream);
+
+/* NLIN_FX (Matrix_Operators.py:743-804): out = F(X,X), [B][3N]. */
+int sddc_nlin_fx(sddc_plan* plan, int B, const double* X, double* out, void* stream);
+
+/* NLIN_DFX (Matrix_Operators.py:807-898): out = F(X,dv) + F(dv,X). */
+int sddc_nlin_dfx(sddc_plan* plan, int B, const double* dv, const double* X,
+                  double* out, void* stream);
+
+/* Back substitutions with the plan's pre-inverted blocks.
+ *   field 0: A4_BSub_TSTEP_V2 (Matrix_Operators.py:1115-1194), dt = Pr*dt
+ *   field 1: NAB2_BSub_TSTEP_V2 (:1033-1086) with Linv_T, dt
+ *   field 2: NAB2_BSub_TSTEP_V2 with Linv_S, Tau*dt
+ * g and f are [B][N] (one field per instance). */
+int sddc_bsub(sddc_plan* plan, int B, int field, const double* g, double* f, void* stream);
+
+/* nsteps applications of Step_Python (Main.py:255-283) with the loop's
+ * X = X_SYM*X_new (Main.py:323).  X_in and X_out are [B][3N]; they may
+ * alias.  linear=1 drops the nonlinear term (Main.py:264). */
+int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
+              const double* X_in, double* X_out, int nsteps, int linear, void* stream);
+
+/* The step linearised about X applied to dv: PDFX without "- dv"
+ * (Main.py:498-521, _ContinC Main.py:804-827). */
+int sddc_step_linearized(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
+                         const double* dv, const double* X, double* out, void* stream);
+
+/* Diagnostics of the time loop (Main.py:292-295): out [B][4] =
+ * (||X||_2, Kinetic_Energy (Main.py:71-134), Nusselt(T) and Nusselt(S)
+ * (Main.py:41-68, value at R_1, no check)). */
+int sddc_diagnostics(sddc_plan* plan, int B, const double* X, double* out, void* stream);
+
+/* Capture nsteps of sddc_step for a fixed batch (pointers fixed) into a HIP
+ * graph; replay with sddc_graph_launch.  No counterpart in the reference:
+ * replaces its Python time loop (Main.py:286-324) for launch-bound runs. */
+typedef struct sddc_graph sddc_graph;
+int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
+                      double* X, int nsteps, void* stream, sddc_graph** out);
+int sddc_graph_launch(sddc_graph* g, void* stream);
+int sddc_graph_destroy(sddc_graph* g);
+
+const char* sddc_last_error(void);
+const char* sddc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDDC_H */

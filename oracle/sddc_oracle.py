@@ -1,0 +1,602 @@
+"""CPU oracle for the pseudospectral double-diffusive time step.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
+checker or as the timed CPU baseline.  The product package
+(``spectraldoublediffusiveconvection_amd``) never imports it.
+
+It restates, in NumPy/SciPy float64, the algorithm of the reference
+(mannixp/SpectralDoubleDiffusiveConvection) for the time-stepping hot path.
+Every function cites the reference file:line it follows.
+
+State layout (Matrix_Operators.py:529-627): ``X = [psi | T | S]``; each field
+has ``N = N_fm*nr`` values stored mode-major, ``X[f*N + k*nr + i]`` with
+``nr = N_r - 1`` interior Chebyshev points.  psi row k <-> sin((k+1) theta);
+T/S row k <-> cos(k theta).
+
+How each part is pinned (tests/test_oracle.py):
+  * transforms: the reference's own known-answer tests (Transforms.py:132-374)
+    restated, brute-force O(K*M) sums, scipy round trips;
+  * operators: Chebyshev differentiation exact on polynomials of degree <= N_r,
+    ``L @ L_inv = I``;
+  * theta recurrences: Legendre eigenfunctions P_l(cos theta) and
+    -dP_l/dtheta reduce them to textbook radial operators;
+  * nonlinear terms, step, kinetic energy, Nusselt number: golden vectors
+    written from the reference by tests/golden/make_golden.py, plus
+    bilinearity / directional-derivative / symmetry invariants;
+  * Newton: residual at convergence (scipy ``lgmres`` is third-party; pinned
+    at solution level only).
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.fftpack import dct as _fp_dct, dst as _fp_dst
+
+__all__ = [
+    "grid", "idct", "idst", "dct", "dst",
+    "cheb_radial", "nabla2", "nabla4", "base_state_coeffs", "build_operators",
+    "eq_sym_mask", "j_theta", "dt0_theta", "a2_sine", "a2_sine_r2",
+    "spectral_slabs", "nlin_fx", "nlin_dfx", "nab2_bsub", "a4_bsub",
+    "step", "step_linearized", "kinetic_energy", "nusselt", "time_step",
+    "newton",
+]
+
+
+def _trapz(y, x, axis=-1):
+    f = getattr(np, "trapezoid", None) or np.trapz
+    return f(y, x=x, axis=axis)
+
+
+# --------------------------------------------------------------------------
+# theta transforms (Transforms.py:8-129)
+# --------------------------------------------------------------------------
+def grid(n):
+    """Midpoint grid theta_i = pi*(2i+1)/(2n) on [0, pi] (Transforms.py:8-13)."""
+    dx = np.pi / n
+    return dx * (2.0 * np.arange(n) + 1.0) / 2
+
+
+def idct(a, n=None):
+    """Cosine series sum_k a_k cos(k theta_i) on grid(n); the K input modes are
+    zero-padded to n (Transforms.py:16-26, 116-129: halve a[1:], DCT-III)."""
+    s = np.array(a, dtype=np.float64, copy=True)
+    s[..., 1:] *= 0.5
+    return _fp_dct(s, type=3, n=n, axis=-1)
+
+
+def idst(b, n=None):
+    """Sine series sum_{m=1}^{K-1} b_m sin(m theta_i).  Index m <-> sin(m theta);
+    index 0 is ignored and sin(K theta) is dropped (Transforms.py:41-54, 87-100:
+    halve, shift left by one, zero the last entry, DST-III)."""
+    b = np.asarray(b, dtype=np.float64)
+    s = np.zeros_like(b)
+    s[..., :-1] = 0.5 * b[..., 1:]
+    return _fp_dst(s, type=3, n=n, axis=-1)
+
+
+def dct(x, n=None):
+    """a_k = (2/M) sum_i x_i cos(k theta_i), a_0 halved, truncated to n modes
+    (Transforms.py:28-39, 102-114: DCT-II, times 1/M, [0] *= 0.5)."""
+    y = _fp_dct(np.asarray(x, dtype=np.float64), type=2, axis=-1)
+    y = y * (1 / y.shape[-1])
+    y[..., 0] *= 0.5
+    return y if n is None else y[..., :n]
+
+
+def dst(x, n=None):
+    """b_m = (2/M) sum_i x_i sin(m theta_i), index m <-> sin(m theta), b_0 = 0,
+    truncated to n (Transforms.py:56-70, 73-85: DST-II, shift right, times 1/M)."""
+    y = _fp_dst(np.asarray(x, dtype=np.float64), type=2, axis=-1)
+    out = np.empty_like(y)
+    out[..., 1:] = y[..., :-1]
+    out[..., 0] = 0.0
+    out *= 1 / y.shape[-1]
+    return out if n is None else out[..., :n]
+
+
+# --------------------------------------------------------------------------
+# radial operators (Matrix_Operators.py:10-76, Main.py:23-38, 179-225)
+# --------------------------------------------------------------------------
+def cheb_radial(N, d):
+    """Chebyshev-Gauss-Lobatto points mapped to [1/d, (1+d)/d] (ascending) and
+    the collocation derivative with negative-row-sum diagonal
+    (Matrix_Operators.py:10-28; Trefethen's cheb)."""
+    r_i = 1.0 / d
+    r_o = (1.0 + d) / d
+    n = np.arange(0, N + 1)
+    x = np.cos(np.pi * n / N).reshape(N + 1, 1)
+    x = 0.5 * (r_i + r_o) + 0.5 * (r_i - r_o) * x
+    c = (np.hstack(([2.0], np.ones(N - 1), [2.0])) * (-1) ** n).reshape(N + 1, 1)
+    xx = np.tile(x, (1, N + 1))
+    dx = xx - xx.T
+    D = np.dot(c, 1.0 / c.T) / (dx + np.eye(N + 1))
+    D -= np.diag(np.sum(D.T, axis=0))
+    return D, x.reshape(N + 1)
+
+
+def nabla2(D, r):
+    """Interior block of r^2 d^2/dr^2 + 2r d/dr (Matrix_Operators.py:31-45)."""
+    A = np.diag(r[:] ** 2) @ (D @ D) + np.diag(2.0 * r[:]) @ D
+    return A[1:-1, 1:-1]
+
+
+def nabla4(D, r):
+    """Clamped d^4/dr^4 through S = diag(1/((r-r_i)(r-r_o))) (Matrix_Operators.py:48-76)."""
+    one = np.ones(len(r))
+    b = -(r[0] + r[-1])
+    c = r[0] * r[-1]
+    q = (r ** 2) + b * r + c * one
+    with np.errstate(divide="ignore"):
+        S = np.diag(1.0 / q)
+    S[0, 0] = 0.0
+    S[-1, -1] = 0.0
+    D2 = D @ D
+    D3 = D @ D2
+    D4 = D2 @ D2
+    L4 = np.diag(q) @ D4 + 4.0 * np.diag(2.0 * r + b * one) @ D3 + 12.0 * D2
+    return (L4 @ S)[1:-1, 1:-1]
+
+
+def base_state_coeffs(d):
+    """T0 = -A_T/r + B_T (Main.py:23-38)."""
+    R_1 = 1.0 / d
+    R_2 = (1.0 + d) / d
+    return (R_1 * R_2) / (R_1 - R_2), R_1 / (R_1 - R_2)
+
+
+class Operators:
+    """Everything Build_Matrix_Operators assembles (Main.py:179-225), plus the
+    per-function matrices the theta operators build internally."""
+
+
+def build_operators(N_fm, N_r, d, dt, Pr, Tau):
+    """Main.py:179-225, A4_TSTEP_MATS (Matrix_Operators.py:1089-1112),
+    NAB2_TSTEP_MATS (1014-1030), R2 (79-94), kGR_RT (97-128)."""
+    o = Operators()
+    D, R = cheb_radial(N_r, d)
+    nr = N_r - 1
+    K = N_fm
+    ri = R[1:-1]
+    o.N_fm, o.N_r, o.nr, o.d, o.dt, o.Pr, o.Tau = K, N_r, nr, d, dt, Pr, Tau
+    o.D, o.R = D, R
+    o.Dr = np.ascontiguousarray(D[1:-1, 1:-1])
+    A_T, _ = base_state_coeffs(d)
+    o.DT0 = A_T / (ri ** 2)
+    o.Rsq = ri ** 2                                   # R2(): block-diag r^2
+    R_1 = 1.0 / d
+    GR = (R_1 ** 2) / (ri ** 2)
+    # kGR_RT: psi row jj <- T row jj+1 with coefficient -(jj+1)*GR
+    o.grk = np.array([-(jj + 1) * GR for jj in range(K)])
+
+    # A4 pieces (Matrix_Operators.py:1093-1101, Main.py:209-214)
+    IR = np.diag(1.0 / R)
+    IR2f = IR @ IR
+    Dsq = D @ D
+    D4 = nabla4(D, R)
+    o.D2_A4 = np.ascontiguousarray((IR2f @ (2 * Dsq - 4 * (IR @ D) + 6 * IR2f))[1:-1, 1:-1])
+    A2 = Dsq[1:-1, 1:-1]
+    IR2 = IR2f[1:-1, 1:-1]
+    IR4 = IR2 @ IR2
+    o.IR2_A4 = np.ascontiguousarray(IR2)
+    o.IR4_A4 = np.ascontiguousarray(IR4)
+    pdt = Pr * dt
+    inv_a4 = []
+    for jj in range(K):
+        j = K - jj
+        bj = -j * (j + 1)
+        L1 = o.D2_A4 + bj * IR4
+        L = (A2 + bj * IR2) - pdt * (D4 + bj * L1)
+        inv_a4.append(np.linalg.inv(L))
+    o.Linv_A4 = np.ascontiguousarray(np.array(inv_a4))
+
+    # NAB2 inverses for T (dt) and S (Tau*dt)
+    I = np.eye(nr)
+    R2m = np.diag(ri ** 2)
+    R2_Nab2 = nabla2(D, R)
+
+    def nab2_mats(h):
+        out = []
+        for jj in range(K):
+            j = K - (jj + 1)
+            bj = -j * (j + 1)
+            A = R2m - h * (R2_Nab2 + bj * I)
+            out.append(np.linalg.inv(A))
+        return np.ascontiguousarray(np.array(out))
+
+    o.Linv_T = nab2_mats(dt)
+    o.Linv_S = nab2_mats(Tau * dt)
+
+    # operators built inside A2_SINE (Matrix_Operators.py:216-218) and
+    # A2_SINE_R2 (Matrix_Operators.py:496-499)
+    o.IR2v = 1.0 / (ri ** 2)
+    o.D2s = np.ascontiguousarray((D @ D)[1:-1, 1:-1])
+    o.IR4v = 1.0 / (ri ** 4)
+    o.D2r2 = np.ascontiguousarray((np.diag(1.0 / R ** 2) @ (D @ D))[1:-1, 1:-1])
+    return o
+
+
+def eq_sym_mask(K, nr):
+    """Equatorial-symmetry 0/1 mask (Main.py:137-176): psi rows with even k and
+    T/S rows with odd k are zeroed."""
+    N = K * nr
+    m = np.ones(3 * N)
+    for k in range(K):
+        if k % 2 == 0:
+            m[k * nr:(k + 1) * nr] = 0.0
+        else:
+            m[N + k * nr:N + (k + 1) * nr] = 0.0
+            m[2 * N + k * nr:2 * N + (k + 1) * nr] = 0.0
+    return m
+
+
+# --------------------------------------------------------------------------
+# theta-coupled operators (Matrix_Operators.py:131-245, 436-526)
+# --------------------------------------------------------------------------
+def _suffix_same_parity(G, inclusive):
+    """Running sums over rows of the same parity, accumulated from the highest
+    row down exactly as the reference loops do.  Row r gets
+    sum_{m >= r (inclusive) or m > r, m = r mod 2} G[m]."""
+    K = G.shape[0]
+    out = np.zeros_like(G)
+    for p in (0, 1):
+        rows = np.arange(p, K, 2)[::-1]
+        if rows.size == 0:
+            continue
+        cs = np.cumsum(G[rows], axis=0)
+        if inclusive:
+            out[rows] = cs
+        else:
+            out[rows[1:]] = cs[:-1]
+    return out
+
+
+def j_theta(g, K, nr):
+    """(1/sin)d_theta(sin * psi): sine rows -> cosine rows (Matrix_Operators.py:436-472).
+    f_j = (j+1) g_{j-1} + 2 sum_{m>=j+1, m=j+1 mod 2} g_m ;  f_0 = sum_{m odd} g_m."""
+    G = np.asarray(g, dtype=np.float64).reshape(K, nr)
+    Q = np.zeros((K + 1, nr))
+    Q[:K] = _suffix_same_parity(G, inclusive=True)
+    F = np.empty((K, nr))
+    F[0] = 1.0 * Q[1]
+    jp1 = (np.arange(1, K) + 1.0)[:, None]
+    F[1:] = jp1 * G[:-1] + 2.0 * Q[2:K + 1]
+    return F.reshape(-1)
+
+
+def dt0_theta(g, DT0, K, nr):
+    """r^2 u_r dT0/dr term = DT0(r) * J_theta(psi) (Matrix_Operators.py:131-189)."""
+    return (j_theta(g, K, nr).reshape(K, nr) * DT0[None, :]).reshape(-1)
+
+
+def a2_sine(g, o):
+    """E^2 psi in the sine basis (Matrix_Operators.py:192-245):
+    f_r = D2 g_r - j IR2 ((j+1) g_r + 2 sum_{m>r, same parity} g_m), j = r+1."""
+    K, nr = o.N_fm, o.nr
+    G = np.asarray(g, dtype=np.float64).reshape(K, nr)
+    P = _suffix_same_parity(G, inclusive=False)
+    j = np.arange(1, K + 1, dtype=np.float64)[:, None]
+    F = G @ o.D2s.T - (j * o.IR2v[None, :]) * ((j + 1) * G + 2.0 * P)
+    return F.reshape(-1)
+
+
+def a2_sine_r2(g, o):
+    """(1/r^2) E^2 psi (Matrix_Operators.py:475-526): as a2_sine with
+    D2 -> diag(1/R^2) D^2 and IR2 -> 1/r^4."""
+    K, nr = o.N_fm, o.nr
+    G = np.asarray(g, dtype=np.float64).reshape(K, nr)
+    P = _suffix_same_parity(G, inclusive=False)
+    j = np.arange(1, K + 1, dtype=np.float64)[:, None]
+    F = G @ o.D2r2.T - j * (o.IR4v[None, :] * ((j + 1) * G + 2.0 * P))
+    return F.reshape(-1)
+
+
+# --------------------------------------------------------------------------
+# nonlinear advection (Matrix_Operators.py:630-898)
+# --------------------------------------------------------------------------
+def _to_sinusoid(A):
+    """(K, nr) sine rows (row k <-> sin((k+1)theta)) -> (nr, K) slab whose
+    column m <-> sin(m theta); column 0 is zero and row K-1 is dropped
+    (Matrix_Operators.py:734-737)."""
+    K, nr = A.shape
+    out = np.zeros((nr, K))
+    out[:, 1:] = A[:-1].T
+    return out
+
+
+def spectral_slabs(X, o):
+    """The nine (nr, N_fm) spectral slabs of Derivatives (Matrix_Operators.py:630-740),
+    with J_theta_RT and A2_SINE_R2 of psi computed first (762-763)."""
+    K, nr = o.N_fm, o.nr
+    N = K * nr
+    PSI = X[:N].reshape(K, nr)
+    T = X[N:2 * N].reshape(K, nr)
+    S = X[2 * N:3 * N].reshape(K, nr)
+    JPSI = j_theta(X[:N], K, nr).reshape(K, nr)
+    OMEGA = a2_sine_r2(X[:N], o).reshape(K, nr)
+    ks = np.arange(1, K + 1)[:, None]
+    kc = np.arange(K)[:, None]
+    Dpsi = PSI @ o.Dr.T
+    return {
+        "JT": np.ascontiguousarray(JPSI.T),
+        "kDpsi": _to_sinusoid(ks * Dpsi),
+        "komega": _to_sinusoid(ks * OMEGA),
+        "DT": np.ascontiguousarray((T @ o.Dr.T).T),
+        "DC": np.ascontiguousarray((S @ o.Dr.T).T),
+        "omega": _to_sinusoid(OMEGA),
+        "Dpsi": _to_sinusoid(Dpsi),
+        "kT": np.ascontiguousarray(((-kc) * T).T),
+        "kC": np.ascontiguousarray(((-kc) * S).T),
+    }
+
+
+def _grid_fields(s, M):
+    g = {}
+    for key in ("JT", "komega", "kDpsi", "DT", "DC"):
+        g[key] = idct(s[key], n=M)
+    for key in ("omega", "Dpsi", "kT", "kC"):
+        g[key] = idst(s[key], n=M)
+    return g
+
+
+def _project(Npsi, NT, NC, K):
+    """Forward transforms, truncation to K modes and the shift of psi back to
+    the code convention (Matrix_Operators.py:797-804)."""
+    Jpsi = dst(Npsi)[:, 0:K]
+    JT = dct(NT)[:, 0:K]
+    JC = dct(NC)[:, 0:K]
+    Jpsi[:, 0:-1] = Jpsi[:, 1:].copy()
+    Jpsi[:, -1] = 0.0
+    return np.concatenate((Jpsi.T.reshape(-1), JT.T.reshape(-1), JC.T.reshape(-1)))
+
+
+def nlin_fx(X, o, symmetric=False):
+    """F(X, X): nonlinear advection terms (Matrix_Operators.py:743-804).
+    symmetric=True reads only the symmetric rows and writes only those
+    (equivalent to masking input and output with eq_sym_mask)."""
+    K, nr = o.N_fm, o.nr
+    if K % 2 != 0:
+        raise ValueError("The number of Fourier modes is not even %d" % K)
+    X = np.asarray(X, dtype=np.float64)
+    if symmetric:
+        X = X * eq_sym_mask(K, nr)
+    M = (3 * K) // 2
+    g = _grid_fields(spectral_slabs(X, o), M)
+    Npsi = o.Dr @ (g["JT"] * g["omega"]) - (g["kDpsi"] * g["omega"] + g["Dpsi"] * g["komega"])
+    NT = g["JT"] * g["DT"] - g["Dpsi"] * g["kT"]
+    NC = g["JT"] * g["DC"] - g["Dpsi"] * g["kC"]
+    out = _project(Npsi, NT, NC, K)
+    if symmetric:
+        out *= eq_sym_mask(K, nr)
+    return out
+
+
+def nlin_dfx(dv, X, o, symmetric=False):
+    """F(X, dv) + F(dv, X): Frechet derivative of nlin_fx (Matrix_Operators.py:807-898)."""
+    K, nr = o.N_fm, o.nr
+    if K % 2 != 0:
+        raise ValueError("The number of Fourier modes is not even %d" % K)
+    X = np.asarray(X, dtype=np.float64)
+    dv = np.asarray(dv, dtype=np.float64)
+    if symmetric:
+        m = eq_sym_mask(K, nr)
+        X = X * m
+        dv = dv * m
+    M = (3 * K) // 2
+    a = _grid_fields(spectral_slabs(X, o), M)
+    b = _grid_fields(spectral_slabs(dv, o), M)
+    Npsi = o.Dr @ (a["JT"] * b["omega"]) - (a["kDpsi"] * b["omega"] + a["Dpsi"] * b["komega"])
+    Npsi += o.Dr @ (b["JT"] * a["omega"]) - (b["kDpsi"] * a["omega"] + b["Dpsi"] * a["komega"])
+    NT = (b["JT"] * a["DT"] - b["Dpsi"] * a["kT"]) + (a["JT"] * b["DT"] - a["Dpsi"] * b["kT"])
+    NC = (b["JT"] * a["DC"] - b["Dpsi"] * a["kC"]) + (a["JT"] * b["DC"] - a["Dpsi"] * b["kC"])
+    out = _project(Npsi, NT, NC, K)
+    if symmetric:
+        out *= eq_sym_mask(K, nr)
+    return out
+
+
+# --------------------------------------------------------------------------
+# implicit solves (Matrix_Operators.py:1033-1086, 1115-1194)
+# --------------------------------------------------------------------------
+def nab2_bsub(g, Linv, K, nr, dt, symmetric=False):
+    """Solve r^2(1 - dt Lap) f = g by back substitution over cosine rows, high
+    to low, one chain per parity (Matrix_Operators.py:1033-1086):
+    f_j = Linv[K-1-j] (g_j - c_j b_j),  b_j = sum_{m>j, same parity} 2 dt m f_m,
+    c_0 = 1/2, c_j = 1 otherwise."""
+    G = np.asarray(g, dtype=np.float64).reshape(K, nr)
+    F = np.zeros((K, nr))
+    starts = [K - 2] if symmetric else [K - 1, K - 2]
+    for start in starts:
+        b = np.zeros(nr)
+        for j in range(start, -1, -2):
+            if j < K - 2:
+                bk = -2.0 * (j + 2.0)
+                bk = -dt * bk
+                b += bk * F[j + 2]
+            rhs = G[j] - 0.5 * b if j == 0 else G[j] - b
+            F[j] = Linv[K - 1 - j] @ rhs
+    return F.reshape(-1)
+
+
+def a4_bsub(g, Linv, D2, IR4, IR2, K, nr, dt, symmetric=False):
+    """Solve (A^2 - dt A^4) psi = g (dt already times Pr) by back substitution
+    over sine rows with the two running sums f_e, bf_e of the reference
+    (Matrix_Operators.py:1115-1194)."""
+    G = np.asarray(g, dtype=np.float64).reshape(K, nr)
+    F = np.zeros((K, nr))
+    starts = [K - 1] if symmetric else [K - 1, K - 2]
+    for start in starts:
+        f_e = np.zeros(nr)
+        bf_e = np.zeros(nr)
+        for row in range(start, -1, -2):
+            jj = K - 1 - row
+            j = K - jj
+            bj = -j * (j + 1)
+            bjt = -2 * j
+            L1 = D2 + bj * IR4
+            if row < K - 2:
+                f_e += F[row + 2]
+                b = dt * bjt * (L1.dot(f_e) + IR4.dot(bf_e)) - bjt * IR2.dot(f_e)
+                F[row] = Linv[jj].dot(G[row] + b)
+                bf_e += bj * F[row] + bjt * f_e
+            else:
+                F[row] = Linv[jj].dot(G[row])
+                bf_e += bj * F[row]
+    return F.reshape(-1)
+
+
+# --------------------------------------------------------------------------
+# the step (Main.py:255-283) and its linearisation (Main.py:498-521)
+# --------------------------------------------------------------------------
+def _implicit_part(NX, Y, o, Ra, Ra_s, symmetric):
+    """Linear terms and the three solves shared by Step_Python, PFX and PDFX
+    (Main.py:266-283)."""
+    K, nr = o.N_fm, o.nr
+    N = K * nr
+    dt, Pr, Tau = o.dt, o.Pr, o.Tau
+    psi, T, S = Y[0:N], Y[N:2 * N], Y[2 * N:3 * N]
+    psi_T0 = dt0_theta(psi, o.DT0, K, nr)
+    Om = a2_sine(psi, o)
+    v = (Ra * T - Ra_s * S).reshape(K, nr)
+    grv = np.zeros((K, nr))
+    grv[:K - 1] = o.grk[:K - 1] * v[1:]
+    NX[0:N] += Om + dt * Pr * grv.reshape(-1)
+    psi_new = a4_bsub(NX[0:N], o.Linv_A4, o.D2_A4, o.IR4_A4, o.IR2_A4, K, nr, Pr * dt, symmetric)
+    rsq = np.tile(o.Rsq, K)
+    NX[N:2 * N] += rsq * T - dt * psi_T0
+    T_new = nab2_bsub(NX[N:2 * N], o.Linv_T, K, nr, dt, symmetric)
+    NX[2 * N:3 * N] += rsq * S - dt * psi_T0
+    S_new = nab2_bsub(NX[2 * N:3 * N], o.Linv_S, K, nr, Tau * dt, symmetric)
+    return np.concatenate((psi_new, T_new, S_new))
+
+
+def step(X, o, Ra, Ra_s, linear=False, symmetric=False):
+    """One IMEX-Euler step X_{n+1} = Step(X_n) (Main.py:255-283).  With
+    symmetric=True the reference reads only symmetric rows and writes only
+    symmetric rows, which equals masking input and output."""
+    K, nr = o.N_fm, o.nr
+    X = np.asarray(X, dtype=np.float64)
+    if symmetric:
+        X = X * eq_sym_mask(K, nr)
+    if linear:
+        NX = np.zeros(3 * K * nr)
+    else:
+        NX = -1 * o.dt * nlin_fx(X, o)
+    out = _implicit_part(NX, X, o, Ra, Ra_s, symmetric)
+    return out
+
+
+def step_linearized(dv, X, o, Ra, Ra_s, symmetric=False):
+    """Step linearised about X applied to dv (PDFX without the '- dv',
+    Main.py:498-521)."""
+    K, nr = o.N_fm, o.nr
+    dv = np.asarray(dv, dtype=np.float64)
+    if symmetric:
+        m = eq_sym_mask(K, nr)
+        dv = dv * m
+        X = np.asarray(X) * m
+    NX = -1.0 * o.dt * nlin_dfx(dv, X, o)
+    return _implicit_part(NX, dv, o, Ra, Ra_s, symmetric)
+
+
+# --------------------------------------------------------------------------
+# diagnostics (Main.py:41-134)
+# --------------------------------------------------------------------------
+def kinetic_energy(X, R, D, K, nr, symmetric=False):
+    """Volume-averaged kinetic energy (Main.py:71-134): J_theta(psi)/r and
+    Dr psi on grid(3K), trapezoid over the interior radial points, then over
+    theta with sin(theta), times 0.5/V."""
+    N = K * nr
+    psi = np.asarray(X[0:N], dtype=np.float64)
+    psi_j = psi
+    if symmetric:
+        psi_j = psi * eq_sym_mask(K, nr)[0:N]
+    J = j_theta(psi_j, K, nr).reshape(K, nr)
+    if symmetric:
+        J = J * eq_sym_mask(K, nr)[N:2 * N].reshape(K, nr)
+    Dr = D[1:-1, 1:-1]
+    IR = 1.0 / R[1:-1]
+    Dpsi_hat = _to_sinusoid(psi.reshape(K, nr) @ Dr.T)
+    Jpsi_hat = IR[:, None] * J.T
+    theta = grid(3 * K)
+    Jpsi = idct(Jpsi_hat, n=3 * K)
+    Dpsi = idst(Dpsi_hat, n=3 * K)
+    KE_rt = Jpsi ** 2 + Dpsi ** 2
+    KE_t = _trapz(KE_rt, R[1:-1], axis=0)
+    KE = _trapz(KE_t * np.sin(theta), theta, axis=-1)
+    V = (2.0 / 3.0) * (R[-1] ** 3 - R[0] ** 3)
+    return (0.5 / V) * KE
+
+
+def nusselt(T_hat, d, R, D, K, nr, check=True):
+    """Nu - 1 at r = R_1 (Main.py:41-68): sum over even cosine modes of
+    D [0, T_k, 0] / (1 - k^2), times R^2 / A_T."""
+    R_1 = 1 / d
+    R_2 = (1 + d) / d
+    A_T = (R_1 * R_2) / (R_1 - R_2)
+    NuM1 = 0.0 * R
+    for k in range(0, K, 2):
+        TT = 0.0 * R
+        TT[1:-1] = T_hat[k * nr:(k + 1) * nr]
+        NuM1 += (D @ TT) / (1.0 - (k ** 2))
+    NuM1 = ((R ** 2) / A_T) * NuM1
+    error = abs(NuM1[-1] - NuM1[0]) / abs(NuM1[-1])
+    if check and error > 1e-03:
+        raise ValueError("|Nu(R_1) - Nu(R_2)|/|Nu(R_1)| = ", error, " > .1%, Increase Nr \n")
+    return NuM1[0]
+
+
+# --------------------------------------------------------------------------
+# drivers
+# --------------------------------------------------------------------------
+def time_step(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric=False, n_steps=1,
+              dt=1e-4, linear=False, diagnostics=True):
+    """The time loop of _Time_Step (Main.py:228-329) without file output:
+    returns the last X and the per-step Norm, KE, Nu_T, Nu_S lists."""
+    o = build_operators(N_fm, N_r, d, dt, Pr, Tau)
+    K, nr = N_fm, N_r - 1
+    N = K * nr
+    mask = eq_sym_mask(K, nr) if symmetric else 1.0
+    hist = {"Norm": [], "KE": [], "Nu_T": [], "Nu_S": []}
+    X = np.asarray(X, dtype=np.float64)
+    X_new = X
+    for _ in range(n_steps):
+        X_new = step(X, o, Ra, Ra_s, linear=linear, symmetric=symmetric)
+        if diagnostics:
+            hist["Norm"].append(np.linalg.norm(X_new, 2))
+            hist["KE"].append(kinetic_energy(X_new, o.R, o.D, K, nr, symmetric))
+            hist["Nu_T"].append(nusselt(X_new[N:2 * N], d, o.R, o.D, K, nr, check=False))
+            hist["Nu_S"].append(nusselt(X_new[2 * N:3 * N], d, o.R, o.D, K, nr, check=False))
+        X = mask * X_new
+    return X_new, hist
+
+
+def newton(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric=False, dt=1,
+           tol_newton=1e-08, tol_gmres=1e-04, Krylov_Space_Size=300):
+    """Matrix-free Newton-Krylov steady solve (Main.py:430-554) with scipy's
+    lgmres on the analytic Jacobian (PDFX).  Returns
+    (X, Norm, KE, NuT, NuS, converged)."""
+    import scipy.sparse.linalg as spla
+    o = build_operators(N_fm, N_r, d, dt, Pr, Tau)
+    K, nr = N_fm, N_r - 1
+    N = K * nr
+    mask = eq_sym_mask(K, nr) if symmetric else 1.0
+    X = np.asarray(X, dtype=np.float64).copy()
+    error, iteration, exit_code = 1.0, 0, 1
+    while error > tol_newton and iteration < 5:
+        X = mask * X
+        fx = step(X, o, Ra, Ra_s, symmetric=symmetric) - X
+        Xc = X.copy()
+        A = spla.LinearOperator((3 * N, 3 * N), dtype="float64",
+                                matvec=lambda v: step_linearized(v, Xc, o, Ra, Ra_s, symmetric) - v)
+        b_norm = np.linalg.norm(fx, 2)
+        dv, exit_code = spla.lgmres(A, fx, maxiter=250, inner_m=Krylov_Space_Size,
+                                    atol=tol_gmres * b_norm)
+        X = X - dv
+        error = np.linalg.norm(dv, 2) / np.linalg.norm(X, 2)
+        iteration += 1
+    if iteration == 5 or exit_code != 0:
+        return None, None, None, None, None, False
+    KE = kinetic_energy(X, o.R, o.D, K, nr, symmetric)
+    NuT = nusselt(X[N:2 * N], d, o.R, o.D, K, nr)
+    NuS = nusselt(X[2 * N:3 * N], d, o.R, o.D, K, nr)
+    return X, np.linalg.norm(X, 2), KE, NuT, NuS, True

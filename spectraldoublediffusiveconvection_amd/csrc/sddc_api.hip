@@ -1,0 +1,532 @@
+// C ABI of libsddc: plan lifecycle, operator upload and the entry points
+// declared in include/sddc.h.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "sddc_internal.h"
+
+struct sddc_plan {
+  sddc::Plan p;
+};
+
+struct sddc_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t own = nullptr;
+};
+
+namespace sddc {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return SDDC_OK;
+  char buf[512];
+  snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  set_error(buf);
+  return SDDC_ERR_HIP;
+}
+
+FftShape make_shape(int L) {
+  FftShape s{};
+  s.L = L;
+  s.npass = 0;
+  int n = L;
+  const int pref[4] = {4, 2, 3, 5};
+  for (int r : pref)
+    while (n % r == 0 && s.npass < kMaxRadices) {
+      s.radix[s.npass++] = r;
+      n /= r;
+    }
+  for (int p = 7; n > 1 && s.npass < kMaxRadices; p += 2)
+    while (n % p == 0) {
+      s.radix[s.npass++] = p;
+      n /= p;
+    }
+  if (n != 1) s.npass = -1;
+  return s;
+}
+
+int build_tables(TableSet& t, int L) {
+  t.L = L;
+  t.shape = make_shape(L);
+  if (t.shape.npass < 0) return fail(SDDC_ERR_UNSUPPORTED, "transform length has too many factors");
+  std::vector<double2> tw(L), rot(L);
+  std::vector<float2> tw32(L), rot32(L);
+  const long double PI = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < L; ++k) {
+    // exp(-2 pi i k/L) and exp(i pi k/(2L)), evaluated in extended precision
+    const long double a = -2.0L * PI * (long double)k / (long double)L;
+    const long double b = PI * (long double)k / (2.0L * (long double)L);
+    tw[k] = make_double2((double)cosl(a), (double)sinl(a));
+    rot[k] = make_double2((double)cosl(b), (double)sinl(b));
+    tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
+    rot32[k] = make_float2((float)rot[k].x, (float)rot[k].y);
+  }
+  SDDC_HIP(hipMalloc(&t.tw64, L * sizeof(double2)));
+  SDDC_HIP(hipMalloc(&t.rot64, L * sizeof(double2)));
+  SDDC_HIP(hipMalloc(&t.tw32, L * sizeof(float2)));
+  SDDC_HIP(hipMalloc(&t.rot32, L * sizeof(float2)));
+  SDDC_HIP(hipMemcpy(t.tw64, tw.data(), L * sizeof(double2), hipMemcpyHostToDevice));
+  SDDC_HIP(hipMemcpy(t.rot64, rot.data(), L * sizeof(double2), hipMemcpyHostToDevice));
+  SDDC_HIP(hipMemcpy(t.tw32, tw32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+  SDDC_HIP(hipMemcpy(t.rot32, rot32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+  return SDDC_OK;
+}
+
+void free_tables(TableSet& t) {
+  if (t.tw64) (void)hipFree(t.tw64);
+  if (t.rot64) (void)hipFree(t.rot64);
+  if (t.tw32) (void)hipFree(t.tw32);
+  if (t.rot32) (void)hipFree(t.rot32);
+  t = TableSet{};
+}
+
+int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
+                    const long long goff[3], double* X, long long xstride,
+                    const long long xoff[3], double* Pout, int fields_mask, hipStream_t s);
+int launch_rhs_scaled(const Plan& p, int B, const void* F, const double* L, double scale,
+                      double* G, hipStream_t s);
+
+// MFMA operand tiling: T[it][kk][lane] = A[16*it + lane%16][4*kk + lane/16].
+template <typename T>
+static void tile_mfma(const double* A, int nr, int NRP, T* out) {
+  const int NT = NRP / 16, NK = NRP / 4;
+  for (int it = 0; it < NT; ++it)
+    for (int kk = 0; kk < NK; ++kk)
+      for (int lane = 0; lane < 64; ++lane) {
+        const int r = 16 * it + (lane & 15), c = 4 * kk + (lane >> 4);
+        const double v = (r < nr && c < nr) ? A[(size_t)r * nr + c] : 0.0;
+        out[((size_t)it * NK + kk) * 64 + lane] = (T)v;
+      }
+}
+
+static void pad_square(const double* A, int nr, int NRP, double* out, bool transpose = false) {
+  for (int r = 0; r < NRP; ++r)
+    for (int c = 0; c < NRP; ++c) {
+      const double v = (r < nr && c < nr) ? A[(size_t)r * nr + c] : 0.0;
+      out[transpose ? (size_t)c * NRP + r : (size_t)r * NRP + c] = v;
+    }
+}
+
+template <typename T>
+static int upload(T** dst, const T* src, size_t n) {
+  SDDC_HIP(hipMalloc(dst, n * sizeof(T)));
+  SDDC_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return SDDC_OK;
+}
+
+static void free_ws(Plan& p) {
+  void* bufs[] = {p.P, p.L, p.G, p.Xa, p.Xb, p.Pa, p.slabs, p.slabs2, p.F, p.rowval, p.params};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  p.P = p.L = p.G = p.Xa = p.Xb = p.Pa = nullptr;
+  p.slabs = p.slabs2 = p.F = nullptr;
+  p.rowval = p.params = nullptr;
+  p.ws_bytes = 0;
+  p.max_batch = 0;
+}
+
+static int alloc_ws(Plan& p, int B) {
+  if (B <= p.max_batch) return SDDC_OK;
+  free_ws(p);
+  const size_t N = (size_t)p.K * p.nr;
+  const size_t rs = p.precision == SDDC_MIXED ? sizeof(float) : sizeof(double);
+  const size_t slab_bytes = (size_t)B * 7 * N * (rs > sizeof(double) ? rs : sizeof(double));
+  size_t total = 0;
+  auto A = [&](void** ptr, size_t bytes) -> int {
+    total += bytes;
+    return hip_check(hipMalloc(ptr, bytes > 0 ? bytes : 8), "workspace");
+  };
+  int rc;
+  if ((rc = A((void**)&p.P, B * N * 8))) return rc;
+  if ((rc = A((void**)&p.Pa, B * N * 8))) return rc;
+  if ((rc = A((void**)&p.L, B * 3 * N * 8))) return rc;
+  if ((rc = A((void**)&p.G, B * 3 * N * 8))) return rc;
+  if ((rc = A((void**)&p.Xa, B * 3 * N * 8))) return rc;
+  if ((rc = A((void**)&p.Xb, B * 3 * N * 8))) return rc;
+  // slabs hold real_t; slabs2 doubles as the float64 slab set of diagnostics
+  if ((rc = A(&p.slabs, slab_bytes))) return rc;
+  if ((rc = A(&p.slabs2, (size_t)B * 7 * N * 8))) return rc;
+  if ((rc = A(&p.F, (size_t)B * 4 * N * rs))) return rc;
+  if ((rc = A((void**)&p.rowval, (size_t)B * p.nr * 8))) return rc;
+  if ((rc = A((void**)&p.params, (size_t)2 * B * 8))) return rc;
+  p.ws_bytes = total;
+  p.max_batch = B;
+  return SDDC_OK;
+}
+
+static int check_batch(Plan& p, int B) {
+  if (B < 0) return fail(SDDC_ERR_INVALID, "negative batch");
+  if (B > p.max_batch) return fail(SDDC_ERR_INVALID, "batch exceeds the plan's reserved size");
+  return SDDC_OK;
+}
+
+}  // namespace sddc
+
+using namespace sddc;
+
+extern "C" {
+
+const char* sddc_last_error(void) { return g_last_error.c_str(); }
+const char* sddc_version(void) { return "sddc 0.1 gfx950"; }
+
+int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric, int max_batch,
+                     sddc_plan** out) {
+  if (!ops || !out) return fail(SDDC_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (ops->N_fm % 2 != 0) {
+    char buf[96];
+    snprintf(buf, sizeof(buf), "The number of Fourier modes is not even %d", ops->N_fm);
+    return fail(SDDC_ERR_ODD_NFM, buf);
+  }
+  if (ops->N_fm < 2 || ops->N_r < 3) return fail(SDDC_ERR_INVALID, "N_fm >= 2 and N_r >= 3 required");
+  if (ops->N_r - 1 > kMaxNr) return fail(SDDC_ERR_UNSUPPORTED, "N_r - 1 above 64 is not supported");
+  if (precision != SDDC_FP64 && precision != SDDC_MIXED)
+    return fail(SDDC_ERR_INVALID, "precision must be SDDC_FP64 or SDDC_MIXED");
+  const double* req[] = {ops->D, ops->R, ops->Dr, ops->D2s, ops->D2r2, ops->IR2v, ops->IR4v,
+                         ops->DT0, ops->Rsq, ops->GR, ops->D2_A4, ops->IR2_A4, ops->IR4_A4,
+                         ops->Linv_A4, ops->Linv_T, ops->Linv_S};
+  for (const double* q : req)
+    if (!q) return fail(SDDC_ERR_INVALID, "null operator pointer");
+  sddc_plan* h = new sddc_plan();
+  Plan& p = h->p;
+  p.K = ops->N_fm;
+  p.N_r = ops->N_r;
+  p.nr = ops->N_r - 1;
+  p.NRP = ((p.nr + 15) / 16) * 16;
+  p.M = 3 * p.K / 2;
+  p.d = ops->d;
+  p.dt = ops->dt;
+  p.Pr = ops->Pr;
+  p.Tau = ops->Tau;
+  p.precision = precision;
+  p.symmetric = symmetric ? 1 : 0;
+  int rc = SDDC_OK;
+  auto bail = [&](int code) {
+    sddc_plan_destroy(h);
+    return code;
+  };
+  if (hipGetDevice(&p.device) != hipSuccess) return bail(fail(SDDC_ERR_HIP, "no HIP device"));
+  int lds_max = 0;
+  (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p.device);
+  if ((size_t)nonlinear_lds_bytes(p, precision, 0) > (size_t)(lds_max > 0 ? lds_max : 65536))
+    return bail(fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel"));
+  const int nr = p.nr, NRP = p.NRP, K = p.K, Np1 = p.N_r + 1;
+  if ((rc = upload(&p.Dfull, ops->D, (size_t)Np1 * Np1))) return bail(rc);
+  if ((rc = upload(&p.R, ops->R, (size_t)Np1))) return bail(rc);
+  const size_t T = (size_t)NRP * NRP;
+  std::vector<double> t64(3 * T);
+  std::vector<float> t32(2 * T);
+  tile_mfma<double>(ops->Dr, nr, NRP, t64.data());
+  tile_mfma<double>(ops->D2r2, nr, NRP, t64.data() + T);
+  tile_mfma<double>(ops->D2s, nr, NRP, t64.data() + 2 * T);
+  tile_mfma<float>(ops->Dr, nr, NRP, t32.data());
+  tile_mfma<float>(ops->D2r2, nr, NRP, t32.data() + T);
+  if ((rc = upload(&p.mfma64, t64.data(), t64.size()))) return bail(rc);
+  if ((rc = upload(&p.mfma32, t32.data(), t32.size()))) return bail(rc);
+  std::vector<double> v((size_t)V_COUNT * NRP, 0.0);
+  for (int i = 0; i < nr; ++i) {
+    v[V_IR2 * NRP + i] = ops->IR2v[i];
+    v[V_IR4 * NRP + i] = ops->IR4v[i];
+    v[V_DT0 * NRP + i] = ops->DT0[i];
+    v[V_RSQ * NRP + i] = ops->Rsq[i];
+    v[V_GR * NRP + i] = ops->GR[i];
+    v[V_IR2A4 * NRP + i] = ops->IR2_A4[i];
+    v[V_IR4A4 * NRP + i] = ops->IR4_A4[i];
+    v[V_IR * NRP + i] = 1. / ops->R[i + 1];
+  }
+  if ((rc = upload(&p.vecs, v.data(), v.size()))) return bail(rc);
+  std::vector<double> sq(T);
+  pad_square(ops->D2_A4, nr, NRP, sq.data());
+  if ((rc = upload(&p.D2A4, sq.data(), T))) return bail(rc);
+  std::vector<double> stack((size_t)K * T);
+  const double* inv[3] = {ops->Linv_A4, ops->Linv_T, ops->Linv_S};
+  double** dst[3] = {&p.LinvA4, &p.LinvT, &p.LinvS};
+  for (int f = 0; f < 3; ++f) {
+    // stored transposed: the solve kernel reads block columns
+    for (int jj = 0; jj < K; ++jj)
+      pad_square(inv[f] + (size_t)jj * nr * nr, nr, NRP, stack.data() + jj * T, true);
+    if ((rc = upload(dst[f], stack.data(), stack.size()))) return bail(rc);
+  }
+  if ((rc = build_tables(p.tabM, p.M))) return bail(rc);
+  if ((rc = build_tables(p.tabKE, 3 * p.K))) return bail(rc);
+  if ((rc = alloc_ws(p, max_batch > 0 ? max_batch : 1))) return bail(rc);
+  *out = h;
+  return SDDC_OK;
+}
+
+int sddc_plan_reserve(sddc_plan* plan, int max_batch) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  SDDC_HIP(hipDeviceSynchronize());
+  return alloc_ws(plan->p, max_batch);
+}
+
+int sddc_plan_destroy(sddc_plan* plan) {
+  if (!plan) return SDDC_OK;
+  Plan& p = plan->p;
+  free_ws(p);
+  void* bufs[] = {p.Dfull, p.R, p.mfma64, p.mfma32, p.vecs, p.D2A4, p.LinvA4, p.LinvT, p.LinvS};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  free_tables(p.tabM);
+  free_tables(p.tabKE);
+  delete plan;
+  return SDDC_OK;
+}
+
+int sddc_plan_info(const sddc_plan* plan, long long* workspace_bytes, int* lds_nonlinear) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  if (workspace_bytes) *workspace_bytes = (long long)plan->p.ws_bytes;
+  if (lds_nonlinear) *lds_nonlinear = nonlinear_lds_bytes(plan->p, plan->p.precision, 0);
+  return SDDC_OK;
+}
+
+int sddc_transform(int kind, const double* in, double* out, int rows, int len_in, int n_out,
+                   void* stream) {
+  if (kind < SDDC_IDCT || kind > SDDC_DST) return fail(SDDC_ERR_INVALID, "unknown transform kind");
+  if (rows < 0 || len_in < 1 || n_out < 1) return fail(SDDC_ERR_INVALID, "bad transform size");
+  const bool inverse = (kind == SDDC_IDCT || kind == SDDC_IDST);
+  if (kind == SDDC_IDST && n_out < len_in)
+    return fail(SDDC_ERR_UNSUPPORTED, "IDST with n below the input length is not supported");
+  const int L = inverse ? n_out : len_in;
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, TableSet> cache;
+  int dev = 0;
+  SDDC_HIP(hipGetDevice(&dev));
+  TableSet* t = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(dev, L);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+      TableSet ts;
+      int rc = build_tables(ts, L);
+      if (rc) return rc;
+      it = cache.emplace(key, ts).first;
+    }
+    t = &it->second;
+  }
+  return launch_transform(*t, kind, rows, len_in, n_out, in, out, (hipStream_t)stream);
+}
+
+static int prep_input(Plan& p, int B, const double* X, const double** use, hipStream_t s,
+                      double* scratch) {
+  *use = X;
+  if (p.symmetric) {
+    int rc = launch_mask(p, B, X, scratch, s);
+    if (rc) return rc;
+    *use = scratch;
+  }
+  return SDDC_OK;
+}
+
+int sddc_theta_ops(sddc_plan* plan, int B, const double* X, double* out, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const double* x;
+  if ((rc = prep_input(p, B, X, &x, s, p.Xa))) return rc;
+  if ((rc = launch_suffix(p, B, x, p.P, s))) return rc;
+  return launch_spectral(p, B, x, p.P, nullptr, nullptr, 0, nullptr, nullptr, SDDC_FP64, out,
+                         nullptr, s);
+}
+
+int sddc_derivatives(sddc_plan* plan, int B, const double* X, double* out, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const double* x;
+  if ((rc = prep_input(p, B, X, &x, s, p.Xa))) return rc;
+  if ((rc = launch_suffix(p, B, x, p.P, s))) return rc;
+  return launch_spectral(p, B, x, p.P, nullptr, nullptr, 0, nullptr, nullptr, SDDC_FP64, nullptr,
+                         out, s);
+}
+
+int sddc_nlin_fx(sddc_plan* plan, int B, const double* X, double* out, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const double* x;
+  if ((rc = prep_input(p, B, X, &x, s, p.Xa))) return rc;
+  if ((rc = launch_suffix(p, B, x, p.P, s))) return rc;
+  if ((rc = launch_spectral(p, B, x, p.P, nullptr, nullptr, 0, nullptr, p.slabs, p.precision,
+                            nullptr, nullptr, s)))
+    return rc;
+  if ((rc = launch_nonlinear(p, B, p.slabs, nullptr, p.F, s))) return rc;
+  if ((rc = launch_rhs_scaled(p, B, p.F, nullptr, 1.0, out, s))) return rc;
+  if (p.symmetric) return launch_mask(p, B, out, out, s);
+  return SDDC_OK;
+}
+
+static int dfx_pipeline(Plan& p, int B, const double* dv, const double* X, const double* Ra,
+                        const double* Ras, bool linear_terms, double scale, double* out,
+                        hipStream_t s) {
+  int rc;
+  const double *x = X, *v = dv;
+  if (p.symmetric) {
+    if ((rc = launch_mask(p, B, X, p.Xa, s))) return rc;
+    if ((rc = launch_mask(p, B, dv, p.Xb, s))) return rc;
+    x = p.Xa;
+    v = p.Xb;
+  }
+  if ((rc = launch_suffix(p, B, x, p.P, s))) return rc;
+  if ((rc = launch_spectral(p, B, x, p.P, nullptr, nullptr, 0, nullptr, p.slabs, p.precision,
+                            nullptr, nullptr, s)))
+    return rc;
+  if ((rc = launch_suffix(p, B, v, p.Pa, s))) return rc;
+  // the second slab set reuses slabs2 in the nonlinear pipeline precision
+  if ((rc = launch_spectral(p, B, v, p.Pa, Ra, Ras, linear_terms ? 1 : 0, p.L, p.slabs2,
+                            p.precision, nullptr, nullptr, s)))
+    return rc;
+  if ((rc = launch_nonlinear(p, B, p.slabs, p.slabs2, p.F, s))) return rc;
+  return launch_rhs_scaled(p, B, p.F, linear_terms ? p.L : nullptr, scale, out, s);
+}
+
+int sddc_nlin_dfx(sddc_plan* plan, int B, const double* dv, const double* X, double* out,
+                  void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = dfx_pipeline(p, B, dv, X, nullptr, nullptr, false, 1.0, out, s))) return rc;
+  if (p.symmetric) return launch_mask(p, B, out, out, s);
+  return SDDC_OK;
+}
+
+int sddc_bsub(sddc_plan* plan, int B, int field, const double* g, double* f, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  if (field < 0 || field > 2) return fail(SDDC_ERR_INVALID, "field must be 0 (psi), 1 (T) or 2 (S)");
+  const long long N = (long long)p.K * p.nr;
+  const long long off[3] = {0, 0, 0};
+  return launch_solve_ex(p, B, g, N, off, f, N, off, nullptr, 1 << field, (hipStream_t)stream);
+}
+
+static int step_once(Plan& p, int B, const double* Ra, const double* Ras, const double* src,
+                     double* dst, int linear, hipStream_t s) {
+  int rc;
+  if ((rc = launch_spectral(p, B, src, p.P, Ra, Ras, 1, p.L, linear ? nullptr : p.slabs,
+                            p.precision, nullptr, nullptr, s)))
+    return rc;
+  if (!linear && (rc = launch_nonlinear(p, B, p.slabs, nullptr, p.F, s))) return rc;
+  if ((rc = launch_rhs(p, B, p.F, p.L, linear ? 0 : 1, p.G, s))) return rc;
+  return launch_solve(p, B, p.G, dst, p.P, 7, 0, s);
+}
+
+int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s, const double* X_in,
+              double* X_out, int nsteps, int linear, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(SDDC_ERR_INVALID, "negative step count");
+  hipStream_t s = (hipStream_t)stream;
+  if (nsteps == 0) {
+    if (X_out != X_in)
+      SDDC_HIP(hipMemcpyAsync(X_out, X_in, (size_t)B * 3 * p.K * p.nr * sizeof(double),
+                              hipMemcpyDeviceToDevice, s));
+    return SDDC_OK;
+  }
+  const double* src;
+  if ((rc = prep_input(p, B, X_in, &src, s, p.Xa))) return rc;
+  if ((rc = launch_suffix(p, B, src, p.P, s))) return rc;
+  for (int k = 0; k < nsteps; ++k) {
+    if ((rc = step_once(p, B, Ra, Ra_s, src, X_out, linear, s))) return rc;
+    src = X_out;
+  }
+  return SDDC_OK;
+}
+
+int sddc_step_linearized(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
+                         const double* dv, const double* X, double* out, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = dfx_pipeline(p, B, dv, X, Ra, Ra_s, true, -1 * p.dt, p.G, s))) return rc;
+  return launch_solve(p, B, p.G, out, nullptr, 7, 0, s);
+}
+
+int sddc_diagnostics(sddc_plan* plan, int B, const double* X, double* out, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = launch_suffix(p, B, X, p.Pa, s))) return rc;
+  if ((rc = launch_spectral(p, B, X, p.Pa, nullptr, nullptr, 0, nullptr, p.slabs2, SDDC_FP64,
+                            nullptr, nullptr, s)))
+    return rc;
+  if ((rc = launch_ke_rows(p, B, p.slabs2, p.rowval, s))) return rc;
+  return launch_diag_final(p, B, X, p.rowval, out, s);
+}
+
+int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra_s, double* X,
+                      int nsteps, void* stream, sddc_graph** out) {
+  if (!plan || !out) return fail(SDDC_ERR_INVALID, "null argument");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  if (nsteps < 1) return fail(SDDC_ERR_INVALID, "nsteps must be positive");
+  sddc_graph* g = new sddc_graph();
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) {
+    if (hipStreamCreateWithFlags(&g->own, hipStreamNonBlocking) != hipSuccess) {
+      delete g;
+      return fail(SDDC_ERR_HIP, "stream creation failed");
+    }
+    s = g->own;
+  }
+  // run once eagerly so that every lazily set kernel attribute is in place
+  rc = sddc_step(plan, B, Ra, Ra_s, X, p.Xb, 1, 0, s);
+  if (!rc) rc = hip_check(hipStreamSynchronize(s), "warm-up");
+  if (!rc) rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
+  if (!rc) {
+    int rc2 = sddc_step(plan, B, Ra, Ra_s, X, X, nsteps, 0, s);
+    hipError_t e = hipStreamEndCapture(s, &g->graph);
+    rc = rc2 ? rc2 : hip_check(e, "end capture");
+  }
+  if (!rc) rc = hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "instantiate");
+  if (rc) {
+    sddc_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return SDDC_OK;
+}
+
+int sddc_graph_launch(sddc_graph* g, void* stream) {
+  if (!g) return fail(SDDC_ERR_INVALID, "null graph");
+  hipStream_t s = stream ? (hipStream_t)stream : g->own;
+  return hip_check(hipGraphLaunch(g->exec, s), "graph launch");
+}
+
+int sddc_graph_destroy(sddc_graph* g) {
+  if (!g) return SDDC_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->own) (void)hipStreamDestroy(g->own);
+  delete g;
+  return SDDC_OK;
+}
+
+}  // extern "C"

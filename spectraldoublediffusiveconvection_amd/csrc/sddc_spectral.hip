@@ -1,0 +1,530 @@
+// Spectral-space kernels: same-parity suffix sums, the fused "spectral stage"
+// (Chebyshev contractions on MFMA + theta recurrences + linear terms +
+// transposed slab output), right-hand-side assembly, symmetry mask and the
+// diagnostics reduction.
+#include <type_traits>
+
+#include "sddc_internal.h"
+
+namespace sddc {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// P[r] = sum_{m>r, m = r mod 2} psi[m]: the running sum f_e / b of A2_SINE,
+// A2_SINE_R2 and J_theta_RT (Matrix_Operators.py:222-231, 441-450, 502-511).
+// One workgroup per instance; threads (parity, i, segment): each segment is
+// summed, segment totals are scanned in LDS, then each segment is rewritten.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_suffix(const double* __restrict__ X,
+                                                 double* __restrict__ P, int K, int nr,
+                                                 int nseg) {
+  extern __shared__ double tot[];  // [nseg][2*nr]
+  const int b = blockIdx.x;
+  const size_t N = (size_t)K * nr;
+  const double* psi = X + (size_t)b * 3 * N;
+  double* Pb = P + (size_t)b * N;
+  const int lanes = 2 * nr;
+  const int t = threadIdx.x;
+  const int s = t / lanes;           // segment, 0 = highest rows
+  const int pi = t - s * lanes;
+  const int p = pi / nr, i = pi - p * nr;
+  const int top = ((K - 1) % 2 == p) ? K - 1 : K - 2;   // highest row of parity p
+  const int cnt = top / 2 + 1;                           // rows of parity p
+  const int per = (cnt + nseg - 1) / nseg;
+  const int r_hi = top - 2 * per * s;                    // first row of this segment
+  int nrows = 0;
+  if (s < nseg) {
+    nrows = cnt - per * s;
+    nrows = nrows < 0 ? 0 : (nrows > per ? per : nrows);
+    double acc = 0.0;
+    for (int q = 0; q < nrows; ++q) acc += psi[(size_t)(r_hi - 2 * q) * nr + i];
+    tot[s * lanes + pi] = acc;
+  }
+  __syncthreads();
+  if (s < nseg) {
+    double acc = 0.0;
+    for (int u = 0; u < s; ++u) acc += tot[u * lanes + pi];
+    for (int q = 0; q < nrows; ++q) {
+      const size_t o = (size_t)(r_hi - 2 * q) * nr + i;
+      Pb[o] = acc;
+      acc += psi[o];
+    }
+  }
+}
+
+int launch_suffix(const Plan& p, int B, const double* X, double* P, hipStream_t s) {
+  if (B <= 0) return SDDC_OK;
+  const int lanes = 2 * p.nr;
+  int nseg = 1024 / lanes;
+  const int half = (p.K + 1) / 2;
+  const int want = (half + 7) / 8;   // at least 8 rows per segment
+  if (nseg > want) nseg = want;
+  if (nseg < 1) nseg = 1;
+  int threads = ((nseg * lanes + 63) / 64) * 64;
+  hipLaunchKernelGGL(k_suffix, dim3(B), dim3(threads), nseg * lanes * sizeof(double), s, X, P,
+                     p.K, p.nr, nseg);
+  return hip_check(hipGetLastError(), "k_suffix");
+}
+
+// ---------------------------------------------------------------------------
+// Spectral stage: one workgroup per (tile of 16 modes, instance).
+// ---------------------------------------------------------------------------
+struct SpectralArgs {
+  const double* X;
+  const double* P;
+  const double* Ra;
+  const double* Ras;
+  const double* A64;   // MFMA-tiled Dr | D2r2 | D2s (float64)
+  const float* A32;    // MFMA-tiled Dr | D2r2 (float32)
+  const double* vecs;
+  double* L;           // [b][3][K][nr] or null
+  void* slabs;         // [b][7][nr][K] (real_t) or null
+  double* theta;       // [b][4][K][nr] or null
+  double* slab9;       // [b][9][nr][K] or null
+  int K, nr;
+  double dt, Pr;
+};
+
+template <typename R, int NRP>
+__global__ __launch_bounds__(256) void k_spectral(SpectralArgs a) {
+  constexpr int NT = NRP / 16;
+  constexpr int NK = NRP / 4;
+  __shared__ double sPsi[kTile + 2][NRP];   // modes k0-1 .. k0+16
+  __shared__ double sP[kTile + 1][NRP];     // modes k0 .. k0+16
+  __shared__ double sT[kTile + 1][NRP];
+  __shared__ double sS[kTile + 1][NRP];
+  __shared__ double sC[5][kTile][NRP + 1];  // contraction results [c][mode][i]
+  const int K = a.K, nr = a.nr;
+  const int b = blockIdx.y;
+  const int k0 = blockIdx.x * kTile;
+  const size_t N = (size_t)K * nr;
+  const double* Xb = a.X + (size_t)b * 3 * N;
+  const double* Pb = a.P + (size_t)b * N;
+
+  for (int idx = threadIdx.x; idx < (kTile + 2) * NRP; idx += blockDim.x) {
+    const int t = idx / NRP, i = idx - t * NRP;
+    const int k = k0 - 1 + t;
+    const bool ok = (k >= 0 && k < K && i < nr);
+    const size_t o = (size_t)k * nr + i;
+    sPsi[t][i] = ok ? Xb[o] : 0.0;
+    if (t >= 1) {
+      sP[t - 1][i] = ok ? Pb[o] : 0.0;
+      sT[t - 1][i] = ok ? Xb[N + o] : 0.0;
+      sS[t - 1][i] = ok ? Xb[2 * N + o] : 0.0;
+    }
+  }
+  __syncthreads();
+
+  // Five contractions (nr x nr) @ (nr x 16 modes).  Both MFMA shapes take
+  // A[i][k] from lane i + 16k and B[k][j] from lane j + 16k; the output
+  // layouts differ between float64 and float32 (see the stores below).
+  //  0 Dr psi, 1 D2r2 psi, 2 D2s psi (float64 always: linear term), 3 Dr T, 4 Dr S
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  for (int task = wave; task < 5 * NT; task += 4) {
+    const int c = task / NT, it = task - c * NT;
+    const double* src = (c <= 2) ? &sPsi[1][0] : (c == 3 ? &sT[0][0] : &sS[0][0]);
+    const double* srow = src + col * NRP + kq;
+    if (c == 2 || std::is_same<R, double>::value) {
+      const int slot = (c == 1) ? 1 : (c == 2 ? 2 : 0);
+      const double* At = a.A64 + ((size_t)(slot * NT + it) * NK) * 64 + lane;
+      double av[NK];
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];   // all operand loads in flight
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], srow[4 * kk], acc, 0, 0, 0);
+      // float64 16x16x4 output: lane l, register v holds row l/16 + 4v
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sC[c][col][16 * it + kq + 4 * v] = acc[v];
+    } else {
+      const int slot = (c == 1) ? 1 : 0;
+      const float* At = a.A32 + ((size_t)(slot * NT + it) * NK) * 64 + lane;
+      float av[NK];
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];
+      v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], (float)srow[4 * kk], acc, 0, 0, 0);
+      // float32 16x16x4 output: lane l, register v holds row 4*(l/16) + v
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sC[c][col][16 * it + 4 * kq + v] = (double)acc[v];
+    }
+  }
+  __syncthreads();
+
+  const double* vIR2 = a.vecs + V_IR2 * NRP;
+  const double* vIR4 = a.vecs + V_IR4 * NRP;
+  const double* vDT0 = a.vecs + V_DT0 * NRP;
+  const double* vRSQ = a.vecs + V_RSQ * NRP;
+  const double* vGR = a.vecs + V_GR * NRP;
+  const double Ra = a.Ra ? a.Ra[b] : 0.0;
+  const double Ras = a.Ras ? a.Ras[b] : 0.0;
+  const double dtpr = a.dt * a.Pr;
+  double* Lb = a.L ? a.L + (size_t)b * 3 * N : nullptr;
+  double* Th = a.theta ? a.theta + (size_t)b * 4 * N : nullptr;
+
+  for (int idx = threadIdx.x; idx < kTile * nr; idx += blockDim.x) {
+    const int rl = idx / nr, i = idx - rl * nr;
+    const int r = k0 + rl;
+    if (r >= K) continue;
+    const double psi = sPsi[rl + 1][i];
+    const double j = r + 1.0;                        // sine wavenumber of row r
+    const double t = (j + 1.0) * psi + 2.0 * sP[rl][i];
+    // A2_SINE_R2 (Matrix_Operators.py:510) and A2_SINE (:230)
+    const double Om = sC[1][rl][i] - j * (vIR4[i] * t);
+    const double A2 = sC[2][rl][i] - (j * vIR2[i]) * t;
+    // J_theta_RT (Matrix_Operators.py:449-455): b = inclusive same-parity sum from row r+1
+    const double Q = sPsi[rl + 2][i] + sP[rl + 1][i];
+    const double JP = (r == 0) ? 1.0 * Q : (r + 1.0) * sPsi[rl][i] + 2.0 * Q;
+    const size_t o = (size_t)r * nr + i;
+    const double psiT0 = vDT0[i] * JP;               // DT0_theta (Matrix_Operators.py:170-172)
+    if (Lb) {
+      // Step_Python right-hand sides (Main.py:271, 276, 280)
+      double grv = 0.0;
+      if (r <= K - 2) {
+        const double v = Ra * sT[rl + 1][i] - Ras * sS[rl + 1][i];
+        grv = ((double)(-(r + 1)) * vGR[i]) * v;     // kGR_RT row r <- cosine row r+1
+      }
+      Lb[o] = A2 + dtpr * grv;
+      Lb[N + o] = vRSQ[i] * sT[rl][i] - a.dt * psiT0;
+      Lb[2 * N + o] = vRSQ[i] * sS[rl][i] - a.dt * psiT0;
+    }
+    if (Th) {
+      Th[o] = JP;
+      Th[N + o] = psiT0;
+      Th[2 * N + o] = A2;
+      Th[3 * N + o] = Om;
+    }
+    sC[1][rl][i] = Om;   // same thread read these entries above
+    sC[2][rl][i] = JP;
+  }
+  __syncthreads();
+
+  if (a.slabs) {
+    const size_t nK = (size_t)nr * K;
+    R* S7 = reinterpret_cast<R*>(a.slabs) + (size_t)b * 7 * nK;
+    for (int idx = threadIdx.x; idx < nr * kTile; idx += blockDim.x) {
+      const int i = idx / kTile, rl = idx - i * kTile;
+      const int r = k0 + rl;
+      if (r >= K) continue;
+      const size_t row = (size_t)i * K;
+      S7[row + r] = (R)sC[2][rl][i];
+      S7[3 * nK + row + r] = (R)sC[3][rl][i];
+      S7[4 * nK + row + r] = (R)sC[4][rl][i];
+      S7[5 * nK + row + r] = (R)sT[rl][i];
+      S7[6 * nK + row + r] = (R)sS[rl][i];
+      if (r + 1 < K) {   // sine rows move to the sinusoid index r+1 (Matrix_Operators.py:734-737)
+        S7[nK + row + r + 1] = (R)sC[0][rl][i];
+        S7[2 * nK + row + r + 1] = (R)sC[1][rl][i];
+      }
+      if (r == 0) {
+        S7[nK + row] = (R)0;
+        S7[2 * nK + row] = (R)0;
+      }
+    }
+  }
+  if (a.slab9) {
+    const size_t nK = (size_t)nr * K;
+    double* S9 = a.slab9 + (size_t)b * 9 * nK;
+    for (int idx = threadIdx.x; idx < nr * kTile; idx += blockDim.x) {
+      const int i = idx / kTile, rl = idx - i * kTile;
+      const int r = k0 + rl;
+      if (r >= K) continue;
+      const size_t row = (size_t)i * K;
+      // order: JT, kDpsi, komega, DT, DC, omega, Dpsi, kT, kC (Matrix_Operators.py:740)
+      S9[row + r] = sC[2][rl][i];
+      S9[3 * nK + row + r] = sC[3][rl][i];
+      S9[4 * nK + row + r] = sC[4][rl][i];
+      S9[7 * nK + row + r] = (double)(-r) * sT[rl][i];
+      S9[8 * nK + row + r] = (double)(-r) * sS[rl][i];
+      if (r + 1 < K) {
+        const double ks = r + 1.0;
+        S9[1 * nK + row + r + 1] = ks * sC[0][rl][i];
+        S9[2 * nK + row + r + 1] = ks * sC[1][rl][i];
+        S9[5 * nK + row + r + 1] = sC[1][rl][i];
+        S9[6 * nK + row + r + 1] = sC[0][rl][i];
+      }
+      if (r == 0) {
+        S9[1 * nK + row] = 0.0;
+        S9[2 * nK + row] = 0.0;
+        S9[5 * nK + row] = 0.0;
+        S9[6 * nK + row] = 0.0;
+      }
+    }
+  }
+}
+
+template <typename R>
+static int launch_spectral_r(const Plan& p, const SpectralArgs& a, int B, hipStream_t s) {
+  const dim3 grid((p.K + kTile - 1) / kTile, B);
+  switch (p.NRP) {
+    case 16: hipLaunchKernelGGL((k_spectral<R, 16>), grid, dim3(256), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_spectral<R, 32>), grid, dim3(256), 0, s, a); break;
+    case 48: hipLaunchKernelGGL((k_spectral<R, 48>), grid, dim3(256), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_spectral<R, 64>), grid, dim3(256), 0, s, a); break;
+    default: return fail(SDDC_ERR_UNSUPPORTED, "unsupported radial size");
+  }
+  return hip_check(hipGetLastError(), "k_spectral");
+}
+
+int launch_spectral(const Plan& p, int B, const double* X, const double* P, const double* Ra,
+                    const double* Ras, int want_linear, double* L, void* slabs,
+                    int slab_precision, double* theta_out, double* slab9, hipStream_t s) {
+  if (B <= 0) return SDDC_OK;
+  SpectralArgs a;
+  a.X = X;
+  a.P = P;
+  a.Ra = Ra;
+  a.Ras = Ras;
+  a.A64 = p.mfma64;
+  a.A32 = p.mfma32;
+  a.vecs = p.vecs;
+  a.L = want_linear ? L : nullptr;
+  a.slabs = slabs;
+  a.theta = theta_out;
+  a.slab9 = slab9;
+  a.K = p.K;
+  a.nr = p.nr;
+  a.dt = p.dt;
+  a.Pr = p.Pr;
+  return slab_precision == SDDC_MIXED ? launch_spectral_r<float>(p, a, B, s)
+                                      : launch_spectral_r<double>(p, a, B, s);
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand side: G = L - dt*F with F_psi = Dr @ F_Q - F_R shifted back to
+// code rows (Matrix_Operators.py:791,797,802; Main.py:262,271,276,280).
+// ---------------------------------------------------------------------------
+struct RhsArgs {
+  const void* F;
+  const double* L;
+  const double* A64;
+  const float* A32;
+  double* G;
+  int K, nr, have_nonlin;
+  double scale;   // -dt for the step (NX = -1*dt*FX, Main.py:262), 1 for F itself
+};
+
+template <typename R, int NRP>
+__global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
+  constexpr int NT = NRP / 16;
+  constexpr int NK = NRP / 4;
+  __shared__ R sQ[kTile][NRP];
+  __shared__ R sR[kTile][NRP];
+  __shared__ R sTT[kTile][NRP];
+  __shared__ R sSS[kTile][NRP];
+  __shared__ R sDQ[kTile][NRP + 1];
+  const int K = a.K, nr = a.nr, b = blockIdx.y, k0 = blockIdx.x * kTile;
+  const size_t N = (size_t)K * nr;
+  const size_t nK = (size_t)nr * K;
+  const double* Lb = a.L ? a.L + (size_t)b * 3 * N : nullptr;
+  double* Gb = a.G + (size_t)b * 3 * N;
+  if (a.have_nonlin) {
+    const R* Fb = reinterpret_cast<const R*>(a.F) + (size_t)b * 4 * nK;
+    for (int idx = threadIdx.x; idx < NRP * kTile; idx += blockDim.x) {
+      const int i = idx / kTile, rl = idx - i * kTile;
+      const int r = k0 + rl, m = r + 1;
+      const bool okm = (i < nr && m < K), okr = (i < nr && r < K);
+      const size_t row = (size_t)i * K;
+      sQ[rl][i] = okm ? Fb[row + m] : (R)0;
+      sR[rl][i] = okm ? Fb[nK + row + m] : (R)0;
+      sTT[rl][i] = okr ? Fb[2 * nK + row + r] : (R)0;
+      sSS[rl][i] = okr ? Fb[3 * nK + row + r] : (R)0;
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int col = lane & 15, kq = lane >> 4;
+    for (int it = wave; it < NT; it += 4) {
+      const R* srow = &sQ[col][kq];
+      if (std::is_same<R, double>::value) {
+        const double* At = a.A64 + ((size_t)it * NK) * 64 + lane;
+        double av[NK];
+#pragma unroll
+        for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < NK; ++kk)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], (double)srow[4 * kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sDQ[col][16 * it + kq + 4 * v] = (R)acc[v];
+      } else {
+        const float* At = a.A32 + ((size_t)it * NK) * 64 + lane;
+        float av[NK];
+#pragma unroll
+        for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < NK; ++kk)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], (float)srow[4 * kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sDQ[col][16 * it + 4 * kq + v] = (R)acc[v];
+      }
+    }
+    __syncthreads();
+  }
+  const double sc = a.scale;
+  for (int idx = threadIdx.x; idx < kTile * nr; idx += blockDim.x) {
+    const int rl = idx / nr, i = idx - rl * nr;
+    const int r = k0 + rl;
+    if (r >= K) continue;
+    const size_t o = (size_t)r * nr + i;
+    if (a.have_nonlin) {
+      // code row r of the psi term is sinusoid index r+1; row K-1 is zero
+      const double Fpsi = (r <= K - 2) ? (double)(sDQ[rl][i] - sR[rl][i]) : 0.0;
+      const double Ft = (double)sTT[rl][i], Fs = (double)sSS[rl][i];
+      if (Lb) {
+        Gb[o] = sc * Fpsi + Lb[o];
+        Gb[N + o] = sc * Ft + Lb[N + o];
+        Gb[2 * N + o] = sc * Fs + Lb[2 * N + o];
+      } else {
+        Gb[o] = sc * Fpsi;
+        Gb[N + o] = sc * Ft;
+        Gb[2 * N + o] = sc * Fs;
+      }
+    } else {
+      Gb[o] = Lb[o];
+      Gb[N + o] = Lb[N + o];
+      Gb[2 * N + o] = Lb[2 * N + o];
+    }
+  }
+}
+
+template <typename R>
+static int launch_rhs_r(const Plan& p, const RhsArgs& a, int B, hipStream_t s) {
+  const dim3 grid((p.K + kTile - 1) / kTile, B);
+  switch (p.NRP) {
+    case 16: hipLaunchKernelGGL((k_rhs<R, 16>), grid, dim3(256), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_rhs<R, 32>), grid, dim3(256), 0, s, a); break;
+    case 48: hipLaunchKernelGGL((k_rhs<R, 48>), grid, dim3(256), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_rhs<R, 64>), grid, dim3(256), 0, s, a); break;
+    default: return fail(SDDC_ERR_UNSUPPORTED, "unsupported radial size");
+  }
+  return hip_check(hipGetLastError(), "k_rhs");
+}
+
+static int rhs_common(const Plan& p, int B, const void* F, const double* L, int have_nonlin,
+                      double scale, double* G, hipStream_t s) {
+  if (B <= 0) return SDDC_OK;
+  RhsArgs a;
+  a.F = F;
+  a.L = L;
+  a.A64 = p.mfma64;
+  a.A32 = p.mfma32;
+  a.G = G;
+  a.K = p.K;
+  a.nr = p.nr;
+  a.have_nonlin = have_nonlin;
+  a.scale = scale;
+  return p.precision == SDDC_MIXED ? launch_rhs_r<float>(p, a, B, s)
+                                   : launch_rhs_r<double>(p, a, B, s);
+}
+
+int launch_rhs(const Plan& p, int B, const void* F, const double* L, int have_nonlin, double* G,
+               hipStream_t s) {
+  return rhs_common(p, B, F, L, have_nonlin, -1 * p.dt, G, s);
+}
+
+int launch_rhs_scaled(const Plan& p, int B, const void* F, const double* L, double scale,
+                      double* G, hipStream_t s) {
+  return rhs_common(p, B, F, L, 1, scale, G, s);
+}
+
+// ---------------------------------------------------------------------------
+// Equatorial symmetry mask (Eq_SYM, Main.py:137-176): psi rows with even k and
+// T/S rows with odd k are zeroed.
+// ---------------------------------------------------------------------------
+__global__ void k_mask(const double* __restrict__ X, double* __restrict__ out, int K, int nr,
+                       size_t total) {
+  const size_t N = (size_t)K * nr;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t w = e % (3 * N);
+    const int f = (int)(w / N);
+    const int k = (int)((w - f * N) / nr);
+    const bool zero = (f == 0) ? (k % 2 == 0) : (k % 2 == 1);
+    out[e] = zero ? 0.0 : X[e];
+  }
+}
+
+int launch_mask(const Plan& p, int B, const double* X, double* out, hipStream_t s) {
+  const size_t total = (size_t)B * 3 * p.K * p.nr;
+  if (total == 0) return SDDC_OK;
+  const int blocks = (int)((total + 255) / 256 < 4096 ? (total + 255) / 256 : 4096);
+  hipLaunchKernelGGL(k_mask, dim3(blocks), dim3(256), 0, s, X, out, p.K, p.nr, total);
+  return hip_check(hipGetLastError(), "k_mask");
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics: ||X||_2, kinetic energy (radial trapezoid over the interior
+// points of the per-row theta integrals, Main.py:129-134) and Nu-1 at R_1 for
+// T and S (Main.py:52-68).  One workgroup per instance.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ X,
+                                                    const double* __restrict__ rowval,
+                                                    const double* __restrict__ Dfull,
+                                                    const double* __restrict__ Rg, int K, int nr,
+                                                    double d, double* __restrict__ out) {
+  __shared__ double red[4][256];
+  const int b = blockIdx.x;
+  const size_t N = (size_t)K * nr;
+  const double* Xb = X + (size_t)b * 3 * N;
+  const int Np1 = nr + 2;
+  double s2 = 0.0, nuT = 0.0, nuS = 0.0, ke = 0.0;
+  for (size_t e = threadIdx.x; e < 3 * N; e += blockDim.x) s2 += Xb[e] * Xb[e];
+  // Nusselt: sum over even k of (D[0,1:-1] . T_k) / (1 - k^2)
+  for (int k = 2 * threadIdx.x; k < K; k += 2 * blockDim.x) {
+    double aT = 0.0, aS = 0.0;
+    for (int i = 0; i < nr; ++i) {
+      const double dd = Dfull[i + 1];
+      aT += dd * Xb[N + (size_t)k * nr + i];
+      aS += dd * Xb[2 * N + (size_t)k * nr + i];
+    }
+    const double den = 1.0 - (double)k * (double)k;
+    nuT += aT / den;
+    nuS += aS / den;
+  }
+  for (int i = threadIdx.x; i < nr; i += blockDim.x) {
+    const double* x = Rg + 1;   // interior points
+    double w;
+    if (nr == 1) w = 0.0;
+    else if (i == 0) w = 0.5 * (x[1] - x[0]);
+    else if (i == nr - 1) w = 0.5 * (x[nr - 1] - x[nr - 2]);
+    else w = 0.5 * (x[i + 1] - x[i - 1]);
+    if (rowval) ke += w * rowval[(size_t)b * nr + i];
+  }
+  red[0][threadIdx.x] = s2;
+  red[1][threadIdx.x] = ke;
+  red[2][threadIdx.x] = nuT;
+  red[3][threadIdx.x] = nuS;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double R1 = 1 / d, R2 = (1 + d) / d;
+    const double A_T = (R1 * R2) / (R1 - R2);
+    const double V = (2.0 / 3.0) * (Rg[Np1 - 1] * Rg[Np1 - 1] * Rg[Np1 - 1] - Rg[0] * Rg[0] * Rg[0]);
+    const double c0 = (Rg[0] * Rg[0]) / A_T;
+    out[(size_t)b * 4 + 0] = sqrt(red[0][0]);
+    out[(size_t)b * 4 + 1] = (0.5 / V) * red[1][0];
+    out[(size_t)b * 4 + 2] = c0 * red[2][0];
+    out[(size_t)b * 4 + 3] = c0 * red[3][0];
+  }
+}
+
+int launch_diag_final(const Plan& p, int B, const double* X, const double* rowval, double* out,
+                      hipStream_t s) {
+  if (B <= 0) return SDDC_OK;
+  hipLaunchKernelGGL(k_diag_final, dim3(B), dim3(256), 0, s, X, rowval, p.Dfull, p.R, p.K, p.nr,
+                     p.d, out);
+  return hip_check(hipGetLastError(), "k_diag_final");
+}
+
+}  // namespace sddc

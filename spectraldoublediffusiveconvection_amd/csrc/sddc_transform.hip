@@ -1,0 +1,456 @@
+// FFT-based kernels: generic transforms (Transforms.py), the fused nonlinear
+// advection kernel (NLIN_FX / NLIN_DFX, Matrix_Operators.py:743-898) and the
+// radial rows of the kinetic energy (Main.py:71-134).
+#include "sddc_fft.h"
+
+namespace sddc {
+
+// ---------------------------------------------------------------------------
+// Generic transforms.  One workgroup transforms two rows with one complex FFT.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_transform(FftShape sh, const double2* tw_,
+                                                   const double2* rot_, int kind, int rows,
+                                                   int len_in, int n_out,
+                                                   const double* __restrict__ in,
+                                                   double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[8];
+  using C = cplx<double>;
+  C* bufA = reinterpret_cast<C*>(smem_raw);
+  C* bufB = bufA + sh.L;
+  const C* tw = reinterpret_cast<const C*>(tw_);
+  const C* rot = reinterpret_cast<const C*>(rot_);
+  const int L = sh.L;
+  const int r0 = 2 * blockIdx.x;
+  const int r1 = r0 + 1;
+  const bool two = r1 < rows;
+  const double* u = in + (size_t)r0 * len_in;
+  const double* v = two ? in + (size_t)r1 * len_in : nullptr;
+  if (kind == SDDC_IDCT || kind == SDDC_IDST) {
+    const bool is_sin = (kind == SDDC_IDST);
+    // IDCT with n < len_in keeps the first n coefficients (DCT-III truncation)
+    const int K = len_in < L ? len_in : L;
+    double mx[2] = {series_absmax<double>(u, K, is_sin, 0), series_absmax<double>(v, K, is_sin, 0)};
+    block_max<2, double>(mx, red);
+    const double su = pow2_scale(mx[0]), sv = pow2_scale(mx[1]);
+    syn_pack<double>(bufA, u, is_sin, 0, v, is_sin, 0, K, L, rot, su, sv);
+    __syncthreads();
+    C* res = fft_run<double>(bufA, bufB, 1, sh, tw, true);
+    const double iu = 1.0 / su, iv = 1.0 / sv;
+    for (int q = threadIdx.x; q < L; q += blockDim.x) {
+      const int n = grid_index(q, L);
+      const double sg = is_sin ? (double)grid_sign(q, L) : 1.0;
+      out[(size_t)r0 * n_out + n] = res[q].x * iu * sg;
+      if (two) out[(size_t)r1 * n_out + n] = res[q].y * iv * sg;
+    }
+  } else {
+    const bool is_sin = (kind == SDDC_DST);
+    double mx[2] = {0.0, 0.0};
+    for (int q = threadIdx.x; q < L; q += blockDim.x) {
+      mx[0] = fmax(mx[0], fabs(u[q]));
+      if (v) mx[1] = fmax(mx[1], fabs(v[q]));
+    }
+    block_max<2, double>(mx, red);
+    const double su = pow2_scale(mx[0]), sv = pow2_scale(mx[1]);
+    for (int q = threadIdx.x; q < L; q += blockDim.x) {
+      const int n = grid_index(q, L);
+      const double sg = is_sin ? (double)grid_sign(q, L) : 1.0;
+      bufA[q] = {su * u[n] * sg, v ? sv * v[n] * sg : 0.0};
+    }
+    __syncthreads();
+    C* res = fft_run<double>(bufA, bufB, 1, sh, tw, false);
+    const int nk = n_out < L ? n_out : L;
+    const double invL = 1.0 / L;
+    const double iu = 1.0 / su, iv = 1.0 / sv;
+    for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+      double c1, c2;
+      if (!is_sin) {
+        ana_unpack<double>(res, L, k, rot, c1, c2);
+        double a1 = (2.0 * (c1 * iu)) * invL, a2 = (2.0 * (c2 * iv)) * invL;
+        if (k == 0) { a1 *= 0.5; a2 *= 0.5; }
+        out[(size_t)r0 * nk + k] = a1;
+        if (two) out[(size_t)r1 * nk + k] = a2;
+      } else {
+        double b1 = 0.0, b2 = 0.0;
+        if (k > 0) {
+          ana_unpack<double>(res, L, L - k, rot, c1, c2);
+          b1 = (2.0 * (c1 * iu)) * invL;
+          b2 = (2.0 * (c2 * iv)) * invL;
+        }
+        out[(size_t)r0 * nk + k] = b1;
+        if (two) out[(size_t)r1 * nk + k] = b2;
+      }
+    }
+  }
+}
+
+int launch_transform(const TableSet& t, int kind, int rows, int len_in, int n_out,
+                     const double* in, double* out, hipStream_t s) {
+  if (rows <= 0) return SDDC_OK;
+  const size_t lds = 2 * (size_t)t.L * sizeof(double2);
+  if (lds > 160 * 1024) return fail(SDDC_ERR_UNSUPPORTED, "transform length too large for LDS");
+  static size_t cur = 0;
+  if (int rc = ensure_dynamic_lds((const void*)k_transform, lds, cur)) return rc;
+  hipLaunchKernelGGL(k_transform, dim3((rows + 1) / 2), dim3(256), lds, s, t.shape, t.tw64,
+                     t.rot64, kind, rows, len_in, n_out, in, out);
+  return hip_check(hipGetLastError(), "k_transform");
+}
+
+// ---------------------------------------------------------------------------
+// Nonlinear advection: one workgroup per (radial row i, instance b).
+// Slab rows (layout [b][7][nr][K]): 0 JT (cos), 1 Dpsi (sine, sinusoid
+// index), 2 Omega (sine), 3 Dr T (cos), 4 Dr S (cos), 5 T (cos row), 6 S.
+// Output rows (layout [b][4][nr][K]): 0 sine coefficients of JT*omega (the
+// radial derivative Dr is applied afterwards in spectral space, where it
+// commutes with the theta transform), 1 sine coefficients of
+// kDpsi*omega + Dpsi*komega, 2 cosine coefficients of the T product,
+// 3 cosine coefficients of the S product (Matrix_Operators.py:791-799).
+// ---------------------------------------------------------------------------
+template <typename R>
+struct SlabRows {
+  const R* s[7];
+};
+
+template <typename R>
+__device__ __forceinline__ SlabRows<R> slab_rows(const R* base, int b, int i, int nr, int K) {
+  SlabRows<R> r;
+#pragma unroll
+  for (int s = 0; s < 7; ++s) r.s[s] = base + ((size_t)(b * 7 + s) * nr + i) * K;
+  return r;
+}
+
+// The nine grid fields as series of the slab rows, in pair order:
+//  0 JT (cos)   1 omega (sine)   | pair 0
+//  2 komega (cos, m*omega_m)     3 Dpsi (sine)     | pair 1
+//  4 kDpsi (cos, m*Dpsi_m)       5 kT (sine, -m T_m) | pair 2
+//  6 DT (cos)   7 kC (sine, -m S_m)                  | pair 3
+//  8 DC (cos)                                        | pair 4
+template <typename R>
+__device__ __forceinline__ void field_src(const SlabRows<R>& sr, int f, const R*& row, bool& is_sin,
+                                          int& mult) {
+  switch (f) {
+    case 0: row = sr.s[0]; is_sin = false; mult = 0; break;
+    case 1: row = sr.s[2]; is_sin = true; mult = 0; break;
+    case 2: row = sr.s[2]; is_sin = false; mult = 1; break;
+    case 3: row = sr.s[1]; is_sin = true; mult = 0; break;
+    case 4: row = sr.s[1]; is_sin = false; mult = 1; break;
+    case 5: row = sr.s[5]; is_sin = true; mult = 2; break;
+    case 6: row = sr.s[3]; is_sin = false; mult = 0; break;
+    case 7: row = sr.s[6]; is_sin = true; mult = 2; break;
+    default: row = sr.s[4]; is_sin = false; mult = 0; break;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R field_absmax(const SlabRows<R>& sr, int f, int K) {
+  const R* row;
+  bool is_sin;
+  int mult;
+  field_src(sr, f, row, is_sin, mult);
+  return series_absmax<R>(row, K, is_sin, mult);
+}
+
+template <typename R>
+__device__ __forceinline__ void syn_pair(cplx<R>* W, const SlabRows<R>& sr, int p, int K, int L,
+                                         const cplx<R>* rot, const R* sc) {
+  const R *u, *v = nullptr;
+  bool us, vs = false;
+  int um, vm = 0;
+  field_src(sr, 2 * p, u, us, um);
+  if (p < 4) field_src(sr, 2 * p + 1, v, vs, vm);
+  syn_pack<R>(W, u, us, um, v, vs, vm, K, L, rot, sc[2 * p], p < 4 ? sc[2 * p + 1] : (R)1);
+}
+
+template <typename R>
+__device__ void project_and_store(cplx<R>* Z, int L, int K, const cplx<R>* rot, R* __restrict__ F,
+                                  int b, int i, int nr, const R* inv) {
+  // Z[0..L): sine pair (Q, R); Z[L..2L): cosine pair (T, S); inv: unscaling
+  R* fq = F + ((size_t)(b * 4 + 0) * nr + i) * K;
+  R* fr = F + ((size_t)(b * 4 + 1) * nr + i) * K;
+  R* ft = F + ((size_t)(b * 4 + 2) * nr + i) * K;
+  R* fs = F + ((size_t)(b * 4 + 3) * nr + i) * K;
+  const R invL = (R)1 / (R)L;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    R c1, c2;
+    ana_unpack<R>(Z + L, L, k, rot, c1, c2);
+    R a1 = ((R)2 * (c1 * inv[2])) * invL, a2 = ((R)2 * (c2 * inv[3])) * invL;
+    if (k == 0) { a1 *= (R)0.5; a2 *= (R)0.5; }
+    ft[k] = a1;
+    fs[k] = a2;
+    R b1 = (R)0, b2 = (R)0;
+    if (k > 0) {
+      ana_unpack<R>(Z, L, L - k, rot, c1, c2);
+      b1 = ((R)2 * (c1 * inv[0])) * invL;
+      b2 = ((R)2 * (c2 * inv[1])) * invL;
+    }
+    fq[k] = b1;
+    fr[k] = b2;
+  }
+}
+
+// Scale the two product pairs (Q,R) at O and (NT,NS) at O+L to unit
+// magnitude before the forward FFT; returns the unscaling factors.
+template <typename R>
+__device__ __forceinline__ void scale_products(cplx<R>* O, int L, R* red, R (&inv)[4]) {
+  R mx[4] = {(R)0, (R)0, (R)0, (R)0};
+  for (int q = threadIdx.x; q < L; q += blockDim.x) {
+    mx[0] = fmax(mx[0], fabs(O[q].x));
+    mx[1] = fmax(mx[1], fabs(O[q].y));
+    mx[2] = fmax(mx[2], fabs(O[L + q].x));
+    mx[3] = fmax(mx[3], fabs(O[L + q].y));
+  }
+  block_max<4, R>(mx, red);
+  R sc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    sc[k] = pow2_scale(mx[k]);
+    inv[k] = (R)1 / sc[k];
+  }
+  for (int q = threadIdx.x; q < L; q += blockDim.x) {
+    O[q] = {O[q].x * sc[0], O[q].y * sc[1]};
+    O[L + q] = {O[L + q].x * sc[2], O[L + q].y * sc[3]};
+  }
+  __syncthreads();
+}
+
+template <typename R, bool DFX>
+__global__ __launch_bounds__(256) void k_nonlinear(FftShape sh, const cplx<R>* __restrict__ tw_g,
+                                                   const cplx<R>* __restrict__ rot_g, int K, int nr,
+                                                   const R* __restrict__ slabs,
+                                                   const R* __restrict__ slabs2,
+                                                   R* __restrict__ F) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ R red[4 * 18];
+  using C = cplx<R>;
+  const int L = sh.L;
+  C* A = reinterpret_cast<C*>(smem_raw);
+  C* Bf = A + 5 * L;
+  // twiddles and half-sample rotations live in LDS next to the data
+  C* tw = A + 10 * L;
+  C* rot = tw + L;
+  for (int k = threadIdx.x; k < L; k += blockDim.x) {
+    tw[k] = tw_g[k];
+    rot[k] = rot_g[k];
+  }
+  __syncthreads();
+  const int i = blockIdx.x;
+  const int b = blockIdx.y;
+  const SlabRows<R> sr = slab_rows<R>(slabs, b, i, nr, K);
+  SlabRows<R> sd = sr;
+  if (DFX) sd = slab_rows<R>(slabs2, b, i, nr, K);
+
+  // power-of-two scales of the nine fields of X (and of dv)
+  R mx[18];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    mx[f] = field_absmax<R>(sr, f, K);
+    mx[9 + f] = DFX ? field_absmax<R>(sd, f, K) : (R)0;
+  }
+  block_max<18, R>(mx, red);
+  R sc[18], is[18];
+#pragma unroll
+  for (int f = 0; f < 18; ++f) {
+    sc[f] = pow2_scale(mx[f]);
+    is[f] = (R)1 / sc[f];
+  }
+
+  for (int p = 0; p < 5; ++p) syn_pair<R>(A + p * L, sr, p, K, L, rot, sc);
+  __syncthreads();
+  C* G = fft_run<R>(A, Bf, 5, sh, tw, true);   // grid fields of X (scaled)
+  C* O = (G == A) ? Bf : A;                    // free buffer (5 arrays)
+  R inv[4];
+
+  if (!DFX) {
+    for (int q = threadIdx.x; q < L; q += blockDim.x) {
+      const R sg = (R)grid_sign(q, L);
+      const C w0 = G[q], w1 = G[L + q], w2 = G[2 * L + q], w3 = G[3 * L + q], w4 = G[4 * L + q];
+      const R JT = w0.x * is[0], Om = sg * w0.y * is[1];
+      const R kOm = w1.x * is[2], Dpsi = sg * w1.y * is[3];
+      const R kDpsi = w2.x * is[4], kT = sg * w2.y * is[5];
+      const R DT = w3.x * is[6], kC = sg * w3.y * is[7];
+      const R DC = w4.x * is[8];
+      // Matrix_Operators.py:791-793 (Dr applied to JT*omega after projection)
+      const R Q = JT * Om;
+      const R Rr = kDpsi * Om + Dpsi * kOm;
+      const R NT = JT * DT - Dpsi * kT;
+      const R NS = JT * DC - Dpsi * kC;
+      O[q] = {sg * Q, sg * Rr};
+      O[L + q] = {NT, NS};
+    }
+    __syncthreads();
+    scale_products<R>(O, L, red, inv);
+    C* Z = fft_run<R>(O, G, 2, sh, tw, false);
+    project_and_store<R>(Z, L, K, rot, F, b, i, nr, inv);
+    return;
+  }
+
+  // Frechet derivative (Matrix_Operators.py:884-887): every product term is
+  // (field of X) * (field of dv); the dv fields are synthesised one pair at a
+  // time and accumulated.
+  C* Zq = O + 2 * L;   // accumulates (Q, R)
+  C* Zt = O + 3 * L;   // accumulates (NT, NS)
+  for (int q = threadIdx.x; q < L; q += blockDim.x) {
+    Zq[q] = {(R)0, (R)0};
+    Zt[q] = {(R)0, (R)0};
+  }
+  for (int p = 0; p < 5; ++p) {
+    syn_pair<R>(O, sd, p, K, L, rot, sc + 9);
+    __syncthreads();
+    C* Pg = fft_run<R>(O, O + L, 1, sh, tw, true);
+    const R iu = is[9 + 2 * p], iv = p < 4 ? is[9 + 2 * p + 1] : (R)1;
+    for (int q = threadIdx.x; q < L; q += blockDim.x) {
+      const R sg = (R)grid_sign(q, L);
+      const C w0 = G[q], w1 = G[L + q], w2 = G[2 * L + q], w3 = G[3 * L + q], w4 = G[4 * L + q];
+      const R JT = w0.x * is[0], Om = sg * w0.y * is[1], kOm = w1.x * is[2],
+              Dpsi = sg * w1.y * is[3], kDpsi = w2.x * is[4], kT = sg * w2.y * is[5],
+              DT = w3.x * is[6], kC = sg * w3.y * is[7], DC = w4.x * is[8];
+      const R du = Pg[q].x * iu, dvv = sg * Pg[q].y * iv;
+      C zq = Zq[q], zt = Zt[q];
+      if (p == 0) {         // dJT = du, dOm = dvv
+        zq.x += JT * dvv + du * Om;
+        zq.y += kDpsi * dvv;
+        zt.x += du * DT;
+        zt.y += du * DC;
+      } else if (p == 1) {  // dkOm = du, dDpsi = dvv
+        zq.y += Dpsi * du + dvv * kOm;
+        zt.x -= dvv * kT;
+        zt.y -= dvv * kC;
+      } else if (p == 2) {  // dkDpsi = du, dkT = dvv
+        zq.y += du * Om;
+        zt.x -= Dpsi * dvv;
+      } else if (p == 3) {  // dDT = du, dkC = dvv
+        zt.x += JT * du;
+        zt.y -= Dpsi * dvv;
+      } else {              // dDC = du
+        zt.y += JT * du;
+      }
+      if (p == 4) zq = {sg * zq.x, sg * zq.y};
+      Zq[q] = zq;
+      Zt[q] = zt;
+    }
+    __syncthreads();
+  }
+  scale_products<R>(Zq, L, red, inv);
+  C* Z = fft_run<R>(Zq, O, 2, sh, tw, false);
+  project_and_store<R>(Z, L, K, rot, F, b, i, nr, inv);
+}
+
+int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current) {
+  if (bytes <= 64 * 1024 || bytes <= current) return SDDC_OK;
+  const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();   // do not leave the error for the next launch
+    return fail(SDDC_ERR_UNSUPPORTED, "dynamic LDS request above the device limit");
+  }
+  current = bytes;
+  return SDDC_OK;
+}
+
+int nonlinear_lds_bytes(const Plan& p, int precision, int /*dfx*/) {
+  const size_t c = precision == SDDC_MIXED ? sizeof(float2) : sizeof(double2);
+  return (int)(12 * (size_t)p.M * c);   // 2 x 5 working arrays + twiddles + rotations
+}
+
+template <typename R, bool DFX>
+static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
+                       hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
+  const size_t lds = 12 * (size_t)p.M * sizeof(cplx<R>);
+  if (lds + 4 * 18 * sizeof(R) > 160 * 1024)
+    return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
+  static size_t cur = 0;
+  if (int rc = ensure_dynamic_lds((const void*)k_nonlinear<R, DFX>, lds, cur)) return rc;
+  hipLaunchKernelGGL((k_nonlinear<R, DFX>), dim3(p.nr, B), dim3(256), lds, s, p.tabM.shape, tw,
+                     rot, p.K, p.nr, (const R*)slabs, (const R*)slabs2, (R*)F);
+  return hip_check(hipGetLastError(), "k_nonlinear");
+}
+
+int launch_nonlinear(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
+                     hipStream_t s) {
+  if (B <= 0) return SDDC_OK;
+  const bool dfx = slabs2 != nullptr;
+  if (p.precision == SDDC_MIXED) {
+    auto tw = reinterpret_cast<const cplx<float>*>(p.tabM.tw32);
+    auto rot = reinterpret_cast<const cplx<float>*>(p.tabM.rot32);
+    return dfx ? launch_nl_t<float, true>(p, B, slabs, slabs2, F, s, tw, rot)
+               : launch_nl_t<float, false>(p, B, slabs, slabs2, F, s, tw, rot);
+  }
+  auto tw = reinterpret_cast<const cplx<double>*>(p.tabM.tw64);
+  auto rot = reinterpret_cast<const cplx<double>*>(p.tabM.rot64);
+  return dfx ? launch_nl_t<double, true>(p, B, slabs, slabs2, F, s, tw, rot)
+             : launch_nl_t<double, false>(p, B, slabs, slabs2, F, s, tw, rot);
+}
+
+// ---------------------------------------------------------------------------
+// Kinetic energy, one workgroup per (radial row i, instance b): synthesise
+// J_theta(psi)/r (cosine) and Dr psi (sine) on grid(3K), and integrate
+// (J^2 + D^2) sin(theta) over theta with the trapezoid rule
+// (Main.py:107-130).  The radial trapezoid is done in k_diag_final.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ke_rows(FftShape sh, const double2* tw_,
+                                                 const double2* rot_, int K, int nr,
+                                                 const double* __restrict__ slabs,
+                                                 const double* __restrict__ vecs, int NRP,
+                                                 double* __restrict__ rowval) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  using C = cplx<double>;
+  __shared__ double red[256];
+  __shared__ double red2[8];
+  const int L = sh.L;
+  C* A = reinterpret_cast<C*>(smem_raw);
+  C* Bf = A + L;
+  const C* tw = reinterpret_cast<const C*>(tw_);
+  const C* rot = reinterpret_cast<const C*>(rot_);
+  const int i = blockIdx.x, b = blockIdx.y;
+  const double* jt = slabs + ((size_t)(b * 7 + 0) * nr + i) * K;
+  const double* dp = slabs + ((size_t)(b * 7 + 1) * nr + i) * K;
+  // IR@JPSI = (1/R_i) * JPSI (Main.py:102,112)
+  const double ir = vecs[V_IR * NRP + i];
+  double mx[2] = {0.0, 0.0};
+  for (int k = threadIdx.x; k < K; k += blockDim.x) {
+    mx[0] = fmax(mx[0], fabs(ir * jt[k]));
+    if (k > 0) mx[1] = fmax(mx[1], fabs(dp[k]));
+  }
+  block_max<2, double>(mx, red2);
+  const double su = pow2_scale(mx[0]), sv = pow2_scale(mx[1]);
+  for (int k = threadIdx.x; k < L; k += blockDim.x) {
+    const double au = k < K ? (k == 0 ? ir * jt[0] : 0.5 * (ir * jt[k])) : 0.0;
+    const double aum = (k > 0 && L - k < K) ? 0.5 * (ir * jt[L - k]) : 0.0;
+    const int m = L - k;
+    const double av = (k > 0 && m < K) ? 0.5 * dp[m] : 0.0;
+    const double avm = (k > 0 && k < K) ? 0.5 * dp[k] : 0.0;
+    C w = {su * au + sv * avm, sv * av - su * aum};
+    A[k] = cmul(w, rot[k]);
+  }
+  __syncthreads();
+  C* res = fft_run<double>(A, Bf, 1, sh, tw, true);
+  const double iu = 1.0 / su, iv = 1.0 / sv;
+  const double dth = 3.14159265358979323846 / L;
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < L; q += blockDim.x) {
+    const int n = grid_index(q, L);
+    const double th = (3.14159265358979323846 / L) * (2.0 * n + 1.0) / 2;
+    const double J = res[q].x * iu;
+    const double D = res[q].y * iv;  // the (-1)^n sign vanishes in the square
+    const double w = (n == 0 || n == L - 1) ? 0.5 * dth : dth;
+    acc += w * ((J * J + D * D) * sin(th));
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) rowval[(size_t)b * nr + i] = red[0];
+}
+
+int launch_ke_rows(const Plan& p, int B, const void* slabs64, double* rowval, hipStream_t s) {
+  const size_t lds = 2 * (size_t)p.tabKE.L * sizeof(double2);
+  if (lds > 160 * 1024) return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS energy kernel");
+  static size_t cur = 0;
+  if (int rc = ensure_dynamic_lds((const void*)k_ke_rows, lds, cur)) return rc;
+  hipLaunchKernelGGL(k_ke_rows, dim3(p.nr, B), dim3(256), lds, s, p.tabKE.shape, p.tabKE.tw64,
+                     p.tabKE.rot64, p.K, p.nr, (const double*)slabs64, p.vecs, p.NRP, rowval);
+  return hip_check(hipGetLastError(), "k_ke_rows");
+}
+
+}  // namespace sddc

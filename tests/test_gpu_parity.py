@@ -1,0 +1,253 @@
+"""GPU parity of libsddc against the CPU oracle and the reference's golden
+vectors (float64 plain method), plus the mixed-precision bound and the
+size-independent properties at the full workload size.
+
+Tolerances (relative 2-norm unless stated):
+  * transforms, theta operators, slabs, solves: 1e-14
+  * nonlinear terms F(X,X) and F(X,dv)+F(dv,X):   1e-13
+  * one step:                                     1e-12 (the per-step bound)
+  * kinetic energy / Nusselt over a trajectory:   1e-9
+  * mixed precision: per-step state error <= 1e-5, kinetic energy <= 1e-6
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import sddc_oracle as O  # noqa: E402
+from spectraldoublediffusiveconvection_amd import _lib  # noqa: E402
+from spectraldoublediffusiveconvection_amd import device as dev  # noqa: E402
+from spectraldoublediffusiveconvection_amd import operators as ops  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = dict(np.load(os.path.join(HERE, "golden", "golden_c1.npz")))
+C1 = dict(N_r=20, N_fm=64, d=0.35, Pr=1.0, Tau=1.0 / 15.0, Ra=9000.0, Ra_s=500.0, dt=5e-3)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def T(a):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device="cuda")
+
+
+def H(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    o = O.build_operators(C1["N_fm"], C1["N_r"], C1["d"], C1["dt"], C1["Pr"], C1["Tau"])
+    p = dev.Plan.build(C1["N_fm"], C1["N_r"], C1["d"], C1["dt"], C1["Pr"], C1["Tau"], max_batch=4)
+    return o, p
+
+
+def _transform(kind, a, rows, len_in, n_out):
+    out = torch.empty((rows, n_out), dtype=torch.float64, device="cuda")
+    src = T(a)
+    _lib.check(_lib.lib().sddc_transform(kind, ctypes.c_void_p(src.data_ptr()),
+                                         ctypes.c_void_p(out.data_ptr()), rows, len_in, n_out,
+                                         None))
+    return H(out)
+
+
+@pytest.mark.parametrize("K", [64, 10, 16, 48, 256, 512])
+def test_transforms(K):
+    M = 3 * K // 2
+    rng = np.random.default_rng(K)
+    a = rng.standard_normal((7, K)) * np.logspace(0, 3, 7)[:, None]   # rows of unequal scale
+    x = rng.standard_normal((7, M)) * np.logspace(3, 0, 7)[:, None]
+    assert rel(_transform(_lib.SDDC_IDCT, a, 7, K, M), O.idct(a, M)) < 1e-14
+    assert rel(_transform(_lib.SDDC_IDST, a, 7, K, M), O.idst(a, M)) < 1e-14
+    assert rel(_transform(_lib.SDDC_DCT, x, 7, M, K), O.dct(x)[:, :K]) < 1e-14
+    assert rel(_transform(_lib.SDDC_DST, x, 7, M, K), O.dst(x)[:, :K]) < 1e-14
+    # n = None (same length) and the reference KAT sizes
+    assert rel(_transform(_lib.SDDC_IDCT, a, 7, K, K), O.idct(a)) < 1e-14
+    assert rel(_transform(_lib.SDDC_DST, a, 7, K, K), O.dst(a)) < 1e-14
+
+
+def test_theta_ops_and_derivatives(c1):
+    o, p = c1
+    K, nr = C1["N_fm"], C1["N_r"] - 1
+    X = GOLD["X_big"]
+    t = H(p.theta_ops(T(X)))[0]
+    psi = X[:K * nr]
+    assert rel(t[0], O.j_theta(psi, K, nr)) < 1e-14
+    assert rel(t[1], O.dt0_theta(psi, o.DT0, K, nr)) < 1e-14
+    assert rel(t[2], O.a2_sine(psi, o)) < 1e-14
+    assert rel(t[3], O.a2_sine_r2(psi, o)) < 1e-14
+    s = H(p.derivatives(T(X)))[0]
+    for i, n in enumerate(["JT", "kDpsi", "komega", "DT", "DC", "omega", "Dpsi", "kT", "kC"]):
+        assert rel(s[i], GOLD["slab_" + n]) < 1e-14, n
+
+
+def test_nonlinear_terms_vs_golden(c1):
+    _, p = c1
+    assert rel(H(p.nlin_fx(T(GOLD["X_big"])))[0], GOLD["nlin_fx_big_full"]) < 1e-13
+    assert rel(H(p.nlin_fx(T(GOLD["X_small"])))[0], GOLD["nlin_fx_small_full"]) < 1e-13
+    assert rel(H(p.nlin_dfx(T(GOLD["dv"]), T(GOLD["X_big"])))[0], GOLD["nlin_dfx_full"]) < 1e-13
+
+
+def test_back_substitutions(c1):
+    o, p = c1
+    K, nr = C1["N_fm"], C1["N_r"] - 1
+    g = np.random.default_rng(1).standard_normal((2, K * nr))
+    a4 = H(p.bsub(T(g), 0))
+    t = H(p.bsub(T(g), 1))
+    s = H(p.bsub(T(g), 2))
+    for b in range(2):
+        assert rel(a4[b], O.a4_bsub(g[b], o.Linv_A4, o.D2_A4, o.IR4_A4, o.IR2_A4, K, nr,
+                                    C1["Pr"] * C1["dt"])) < 1e-14
+        assert rel(t[b], O.nab2_bsub(g[b], o.Linv_T, K, nr, C1["dt"])) < 1e-14
+        assert rel(s[b], O.nab2_bsub(g[b], o.Linv_S, K, nr, C1["Tau"] * C1["dt"])) < 1e-14
+
+
+def test_single_step_and_linearised(c1):
+    o, p = c1
+    for name in ("X_big", "X_small"):
+        X = GOLD[name]
+        assert rel(H(p.step(T(X), C1["Ra"], C1["Ra_s"]))[0], O.step(X, o, C1["Ra"], C1["Ra_s"])) < 1e-12
+        assert rel(H(p.step(T(X), C1["Ra"], C1["Ra_s"], linear=True))[0],
+                   O.step(X, o, C1["Ra"], C1["Ra_s"], linear=True)) < 1e-12
+    a = H(p.step_linearized(T(GOLD["dv"]), T(GOLD["X_big"]), C1["Ra"], C1["Ra_s"]))[0]
+    assert rel(a, O.step_linearized(GOLD["dv"], GOLD["X_big"], o, C1["Ra"], C1["Ra_s"])) < 1e-12
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_golden_trajectory(sym):
+    """20 steps of the reference driver (_Time_Step) incl. the symmetry mask."""
+    p = dev.Plan.build(C1["N_fm"], C1["N_r"], C1["d"], C1["dt"], C1["Pr"], C1["Tau"], symmetric=sym)
+    tag = "sym" if sym else "full"
+    X = H(p.step(T(GOLD["X_small"]), C1["Ra"], C1["Ra_s"], nsteps=20))[0]
+    assert rel(X, GOLD["traj20_X_" + tag]) < 1e-13
+    d = H(p.diagnostics(T(X)))[0]
+    for j, key in enumerate(("Norm", "KE", "Nu_T", "Nu_S")):
+        assert abs(d[j] / GOLD["traj20_%s_%s" % (key, tag)][-1] - 1) < 1e-12, key
+    if sym:
+        assert rel(H(p.nlin_fx(T(GOLD["X_big"])))[0], GOLD["nlin_fx_big_sym"]) < 1e-13
+        assert rel(H(p.nlin_dfx(T(GOLD["dv"]), T(GOLD["X_big"])))[0], GOLD["nlin_dfx_sym"]) < 1e-13
+
+
+def test_large_state_trajectory_vs_golden(c1):
+    _, p = c1
+    X = H(p.step(T(GOLD["X_big"]), C1["Ra"], C1["Ra_s"], nsteps=10))[0]
+    assert rel(X, GOLD["traj10big_X_full"]) < 1e-12
+    X = H(p.step(T(GOLD["X_small"]), C1["Ra"], C1["Ra_s"], nsteps=10, linear=True))[0]
+    assert rel(X, GOLD["traj10lin_X_full"]) < 1e-13
+
+
+def test_thousand_steps_per_step_parity(c1):
+    """The CPU-runnable workload: 1000 steps from a seeded N(0,1e-3) state,
+    every step compared with the oracle's step of the same input, and the
+    kinetic energy / Nusselt numbers of the final states."""
+    o, p = c1
+    X0 = 1e-3 * np.random.default_rng(2024).standard_normal(3 * C1["N_fm"] * (C1["N_r"] - 1))
+    xg = T(X0).view(1, -1)
+    xc = X0.copy()
+    worst = 0.0
+    for it in range(1000):
+        x_in = H(xg)[0] if it % 10 == 0 else None
+        xg = p.step(xg, C1["Ra"], C1["Ra_s"])
+        if x_in is not None:   # per-step error: the oracle's step of the GPU's own input
+            worst = max(worst, rel(H(xg)[0], O.step(x_in, o, C1["Ra"], C1["Ra_s"])))
+        xc = O.step(xc, o, C1["Ra"], C1["Ra_s"])
+    assert worst < 1e-12, worst
+    assert rel(H(xg)[0], xc) < 1e-9
+    d = H(p.diagnostics(xg))[0]
+    K, nr = C1["N_fm"], C1["N_r"] - 1
+    N = K * nr
+    ke = O.kinetic_energy(xc, o.R, o.D, K, nr)
+    assert abs(d[1] / ke - 1) < 1e-9
+    assert abs(d[2] / O.nusselt(xc[N:2 * N], C1["d"], o.R, o.D, K, nr, check=False) - 1) < 1e-9
+    assert abs(d[3] / O.nusselt(xc[2 * N:], C1["d"], o.R, o.D, K, nr, check=False) - 1) < 1e-9
+
+
+@pytest.mark.parametrize("K,Nr", [(256, 30), (512, 40), (48, 24), (10, 20)])
+def test_step_other_sizes(K, Nr):
+    d, dt, Pr, Tau = 0.35, 5e-3, 1.0, 1 / 15.0
+    o = O.build_operators(K, Nr, d, dt, Pr, Tau)
+    p = dev.Plan(ops.build(K, Nr, d, dt, Pr, Tau))
+    X0 = 1e-3 * np.random.default_rng(K + Nr).standard_normal(3 * K * (Nr - 1))
+    assert rel(H(p.step(T(X0), 9000.0, 500.0))[0], O.step(X0, o, 9000.0, 500.0)) < 1e-12
+    assert rel(H(p.nlin_fx(T(X0)))[0], O.nlin_fx(X0, o)) < 1e-12
+
+
+def test_batched_equals_alone_bitwise(c1):
+    """Instances of one size class stepped together give the same bits as alone."""
+    _, p = c1
+    rng = np.random.default_rng(7)
+    X = 1e-3 * rng.standard_normal((3, 3 * C1["N_fm"] * (C1["N_r"] - 1)))
+    Ra = np.array([8000.0, 9000.0, 11000.0])
+    Ras = np.array([300.0, 500.0, 300.0])
+    both = H(p.step(T(X), Ra, Ras, nsteps=5))
+    for b in range(3):
+        alone = H(p.step(T(X[b]), Ra[b], Ras[b], nsteps=5))[0]
+        assert np.array_equal(both[b], alone)
+
+
+def test_graph_replay_matches_eager(c1):
+    _, p = c1
+    X0 = 1e-3 * np.random.default_rng(9).standard_normal(3 * C1["N_fm"] * (C1["N_r"] - 1))
+    xg = T(X0).view(1, -1).clone()
+    g = p.graph(xg, C1["Ra"], C1["Ra_s"], nsteps=4)
+    eager = T(X0).view(1, -1)
+    for rep in range(3):
+        g.launch()
+        eager = p.step(eager, C1["Ra"], C1["Ra_s"], nsteps=4)
+        assert np.array_equal(H(xg), H(eager)), rep
+    g.close()
+
+
+def test_mixed_precision_error_bound():
+    """Mixed precision at the full workload size N_r=40, N_fm=512: float32
+    nonlinear pipeline, float64 linear terms and solves."""
+    K, Nr, d, dt, Pr, Tau = 512, 40, 0.35, 5e-3, 1.0, 1 / 15.0
+    op = ops.build(K, Nr, d, dt, Pr, Tau)
+    p64 = dev.Plan(op)
+    p32 = dev.Plan(op, precision="mixed")
+    X0 = 1e-3 * np.random.default_rng(11).standard_normal(3 * K * (Nr - 1))
+    x64 = T(X0).view(1, -1)
+    x32 = x64.clone()
+    worst_ke = 0.0
+    for blk in range(10):
+        x64 = p64.step(x64, 9000.0, 500.0, nsteps=100)
+        x32 = p32.step(x32, 9000.0, 500.0, nsteps=100)
+        d64 = H(p64.diagnostics(x64))[0]
+        d32 = H(p64.diagnostics(x32))[0]
+        worst_ke = max(worst_ke, abs(d32[1] / d64[1] - 1))
+    assert worst_ke < 1e-6, worst_ke
+    one64 = H(p64.step(x64, 9000.0, 500.0))[0]
+    one32 = H(p32.step(x64, 9000.0, 500.0))[0]
+    assert rel(one32, one64) < 1e-5
+
+
+def test_full_size_properties():
+    """Size-independent checks at N_r=40, N_fm=512 (no oracle run): F is
+    quadratic and its Frechet derivative satisfies F(X+V)-F(X)-F(V)=DF(X)V."""
+    K, Nr = 512, 40
+    p = dev.Plan(ops.build(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0))
+    rng = np.random.default_rng(5)
+    X = T(1e-2 * rng.standard_normal(3 * K * (Nr - 1)))
+    V = T(1e-2 * rng.standard_normal(3 * K * (Nr - 1)))
+    F = H(p.nlin_fx(X))
+    assert rel(H(p.nlin_fx(2 * X)), 4 * F) < 1e-13
+    lhs = H(p.nlin_fx(X + V)) - F - H(p.nlin_fx(V))
+    assert rel(H(p.nlin_dfx(V, X)), lhs) < 1e-10
+    assert np.abs(H(p.nlin_fx(0 * X))).max() == 0.0
+
+
+def test_errors_are_raised():
+    with pytest.raises(ValueError):
+        dev.Plan.build(63, 20, 0.35, 5e-3, 1.0, 1.0)
+    p = dev.Plan.build(16, 10, 0.35, 5e-3, 1.0, 1.0)
+    with pytest.raises(ValueError):
+        p.step(torch.zeros((1, 17), dtype=torch.float64, device="cuda"), 1.0, 1.0)
+    with pytest.raises(ValueError):
+        p.bsub(torch.zeros((1, 16 * 9), dtype=torch.float64, device="cuda"), 3)
