@@ -1,0 +1,153 @@
+"""Benchmark of the pseudospectral double-diffusive time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+
+A "step" is one batched step of B instances of the size class N_r=40,
+N_fm=512 (d=0.35, Pr=1, Tau=1/15, dt=5e-3) in mixed precision (float32
+transforms, products and Chebyshev GEMMs; float64 linear terms and solves),
+replayed from a HIP graph.  Ra_T is spread over [2500, 3000] and Ra_s is 500
+(the range over which the run stays bounded).  States are
+seeded normal vectors scaled to ||X||_2 = 1e-3, resident in HBM before the
+timed region.  ``value`` is the aggregate instance-steps per second.
+
+Beside it the line reports the plain float64 single-instance step (the
+baseline the metric is quoted against), the roofline of the dominant kernel
+and the CPU baseline (the oracle's NumPy step on one host core, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOAD = dict(N_r=40, N_fm=512, d=0.35, Pr=1.0, Tau=1.0 / 15.0, dt=5e-3, Ra_s=500.0,
+                Ra_lo=2500.0, Ra_hi=3000.0)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def seeded_states(B, n3, seed=1234):
+    X = np.random.default_rng(seed).standard_normal((B, n3))
+    return 1e-3 * X / np.linalg.norm(X, axis=1, keepdims=True)
+
+
+def cpu_baseline(seconds):
+    """The oracle's float64 NumPy step (reference algorithm restated) on one
+    host core, over a bounded sample of the same workload."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import sddc_oracle as O
+    w = WORKLOAD
+    o = O.build_operators(w["N_fm"], w["N_r"], w["d"], w["dt"], w["Pr"], w["Tau"])
+    X = seeded_states(1, 3 * w["N_fm"] * (w["N_r"] - 1))[0]
+    with threadpool_limits(1):
+        X = O.step(X, o, w["Ra_hi"], w["Ra_s"])   # first step untimed (reference timer, Main.py:325)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            X = O.step(X, o, w["Ra_hi"], w["Ra_s"])
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds and n >= 5:
+                break
+    return {"value": n / el, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "%d float64 steps of one N_r=40, N_fm=512 instance (NumPy restatement "
+                      "of the reference step, one thread), %.1f s" % (n, el)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from spectraldoublediffusiveconvection_amd import device as dev
+    from spectraldoublediffusiveconvection_amd import operators as ops
+    from spectraldoublediffusiveconvection_amd import profiling
+
+    w = WORKLOAD
+    B = args.batch
+    op = ops.build(w["N_fm"], w["N_r"], w["d"], w["dt"], w["Pr"], w["Tau"])
+    n3 = 3 * w["N_fm"] * (w["N_r"] - 1)
+    Ra = np.linspace(w["Ra_lo"], w["Ra_hi"], B)
+
+    # ---- batched mixed-precision step, graph-replayed
+    plan = dev.Plan(op, precision="mixed", max_batch=B)
+    X = torch.as_tensor(seeded_states(B, n3), device="cuda")
+    gsteps = max(1, min(args.graph_steps, args.steps))
+    graph = plan.graph(X, Ra, w["Ra_s"], nsteps=gsteps)
+    for _ in range(max(1, args.warmup // gsteps)):
+        graph.launch()
+    torch.cuda.synchronize()
+    nl = max(1, args.steps // gsteps)
+    steps = nl * gsteps
+    t0 = time.perf_counter()
+    for _ in range(nl):
+        graph.launch()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    finite = bool(torch.isfinite(X).all().item())
+    value = B * steps / el
+
+    # ---- per-kernel timing of the dominant kernel (HIP events on its stream)
+    roof = profiling.roofline(plan, X, Ra, w["Ra_s"], reps=20)
+
+    # ---- plain float64 single-instance step (eager launches)
+    p64 = dev.Plan(op)
+    x1 = torch.as_tensor(seeded_states(1, n3, seed=99), device="cuda")
+    out = torch.empty_like(x1)
+    p64.step(x1, w["Ra_hi"], w["Ra_s"], nsteps=5, out=out)
+    torch.cuda.synchronize()
+    n64 = 200
+    t1 = time.perf_counter()
+    p64.step(x1, w["Ra_hi"], w["Ra_s"], nsteps=n64, out=out)
+    torch.cuda.synchronize()
+    el64 = time.perf_counter() - t1
+    plain = n64 / el64
+
+    line = {
+        "metric": "aggregate CN/AB2 steps/s at Nr=40,Ntheta=512 vs plain FP64 GPU; p50/p99 job latency",
+        "value": value,
+        "unit": "instance-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "N_r=40 N_fm=512 d=0.35 Pr=1 Tau=1/15 dt=5e-3 Ra_s=500 "
+                               "Ra_T in [2500,3000]; batched mixed precision, HIP graph",
+                   "batch": B, "graph_steps": gsteps, "states_finite": finite},
+        "plain_fp64_gpu": {"steps_per_s": plain, "ms_per_step": 1e3 * el64 / n64},
+        "speedup_vs_plain_fp64": value / plain,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

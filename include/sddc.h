@@ -132,6 +132,14 @@ int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra
 int sddc_graph_launch(sddc_graph* g, void* stream);
 int sddc_graph_destroy(sddc_graph* g);
 
+/* Launch one stage of the step on the plan's workspace (for timing and
+ * profiling; no counterpart in the reference).  stage: 0 suffix sums,
+ * 1 spectral stage, 2 nonlinear products, 3 right-hand side, 4 solves.
+ * X is the state read by stages 0-1; stage 4 writes into plan scratch.  A
+ * full sddc_step on the same batch must have run first. */
+int sddc_run_stage(sddc_plan* plan, int B, int stage, const double* X, const double* Ra,
+                   const double* Ra_s, void* stream);
+
 const char* sddc_last_error(void);
 const char* sddc_version(void);
 

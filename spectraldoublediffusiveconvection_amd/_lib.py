@@ -62,6 +62,7 @@ _SIGS = {
     "sddc_graph_create": (_I, [_V, _I, _V, _V, _V, _I, _V, ctypes.POINTER(_V)]),
     "sddc_graph_launch": (_I, [_V, _V]),
     "sddc_graph_destroy": (_I, [_V]),
+    "sddc_run_stage": (_I, [_V, _I, _I, _V, _V, _V, _V]),
     "sddc_last_error": (ctypes.c_char_p, []),
     "sddc_version": (ctypes.c_char_p, []),
 }

@@ -480,6 +480,25 @@ int sddc_diagnostics(sddc_plan* plan, int B, const double* X, double* out, void*
   return launch_diag_final(p, B, X, p.rowval, out, s);
 }
 
+int sddc_run_stage(sddc_plan* plan, int B, int stage, const double* X, const double* Ra,
+                   const double* Ra_s, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  switch (stage) {
+    case 0: return launch_suffix(p, B, X, p.P, s);
+    case 1:
+      return launch_spectral(p, B, X, p.P, Ra, Ra_s, 1, p.L, p.slabs, p.precision, nullptr,
+                             nullptr, s);
+    case 2: return launch_nonlinear(p, B, p.slabs, nullptr, p.F, s);
+    case 3: return launch_rhs(p, B, p.F, p.L, 1, p.G, s);
+    case 4: return launch_solve(p, B, p.G, p.Xb, p.Pa, 7, 0, s);
+    default: return fail(SDDC_ERR_INVALID, "stage must be 0..4");
+  }
+}
+
 int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra_s, double* X,
                       int nsteps, void* stream, sddc_graph** out) {
   if (!plan || !out) return fail(SDDC_ERR_INVALID, "null argument");

@@ -58,8 +58,15 @@ __device__ __forceinline__ double dot_row(const double (&Lr)[NRS], const double*
   return (s0 + s1) + (s2 + s3);
 }
 
-// State of one NAB2 chain step: b += 2 dt (j+2) f_{j+2};
-// f_j = Linv_{K-1-j} (g_j - c_j b), c_0 = 1/2 (Matrix_Operators.py:1061-1083).
+// The steps below have no data-dependent branches: the prefetches of the next
+// mode always issue (indices clamped at the last mode) and the first-row cases
+// of the reference are selects.  With straight-line loop bodies the compiler
+// can wait only for the loads of the previous step instead of draining all
+// outstanding loads (s_waitcnt vmcnt(0)) every step.
+
+// One NAB2 step (Matrix_Operators.py:1061-1083): b += 2 dt (j+2) f_{j+2};
+// f_j = Linv_{K-1-j} (g_j - c_j b), c_0 = 1/2.  At the first row of a chain
+// fprev = 0, so the unconditional update adds a signed zero (exact).
 template <int NRS>
 __device__ __forceinline__ void nab2_step(int& j, int K, int nr, int i, bool active, double dtf,
                                           const double* __restrict__ g, double* __restrict__ x,
@@ -67,14 +74,13 @@ __device__ __forceinline__ void nab2_step(int& j, int K, int nr, int i, bool act
                                           int ii, double (&Lc)[NRS], double (&Ln)[NRS],
                                           double& gcur, double& bacc, double& fprev, double* sh) {
   const int jj = K - 1 - j;
-  if (j >= 2) load_row<NRS>(Ln, Linv + (size_t)(jj + 2) * MS, NRP, ii);
+  const int jn = jj + 2 < K ? jj + 2 : K - 1;
+  load_row<NRS>(Ln, Linv + (size_t)jn * MS, NRP, ii);
   const double gv = gcur;
-  gcur = (j >= 2) ? g[(size_t)(j - 2) * nr + ii] : 0.0;   // next mode's right-hand side
-  if (j < K - 2) {
-    double bk = -2.0 * (j + 2.0);
-    bk = -dtf * bk;
-    bacc += bk * fprev;
-  }
+  gcur = g[(size_t)(j >= 2 ? j - 2 : 0) * nr + ii];
+  double bk = -2.0 * (j + 2.0);
+  bk = -dtf * bk;
+  bacc += bk * fprev;
   const double rhs = (j == 0) ? gv - 0.5 * bacc : gv - bacc;
   sh[i] = active ? rhs : 0.0;
   wave_sync();
@@ -85,9 +91,11 @@ __device__ __forceinline__ void nab2_step(int& j, int K, int nr, int i, bool act
   j -= 2;
 }
 
-// One A4 chain step (Matrix_Operators.py:1141-1164), dt = Pr*dt:
+// One A4 step (Matrix_Operators.py:1141-1164), dt = Pr*dt:
 //   f_e += f_{row+2};  b = dt bjt (L1 f_e + IR4 bf_e) - bjt IR2 f_e
 //   f_row = Linv (g_row + b);  bf_e += bj f_row + bjt f_e
+// At the first row of a chain f_e = bf_e = fprev = 0, so b is a signed zero
+// and the updates reduce exactly to the reference's first-row branch.
 template <int NRS>
 __device__ __forceinline__ void a4_step(int& row, int K, int nr, int i, bool active, double pdt,
                                         const double* __restrict__ g, double* __restrict__ x,
@@ -97,44 +105,37 @@ __device__ __forceinline__ void a4_step(int& row, int K, int nr, int i, bool act
                                         double ir4, double ir2, double& gcur, double& fe,
                                         double& bf, double& fprev, double* sh) {
   const int jj = K - 1 - row;
-  if (row >= 2) load_row<NRS>(Ln, Linv + (size_t)(jj + 2) * MS, NRP, ii);
+  const int jn = jj + 2 < K ? jj + 2 : K - 1;
+  load_row<NRS>(Ln, Linv + (size_t)jn * MS, NRP, ii);
   const double gv = gcur;
-  gcur = (row >= 2) ? g[(size_t)(row - 2) * nr + ii] : 0.0;
+  gcur = g[(size_t)(row >= 2 ? row - 2 : 0) * nr + ii];
   const double j = (double)(K - jj);
   const double bj = -j * (j + 1.0);
   const double bjt = -2.0 * j;
-  double y;
-  if (row < K - 2) {
-    fe += fprev;
-    sh[i] = active ? fe : 0.0;
-    wave_sync();
-    const double diag = d2ii + bj * ir4;   // L1 = D2 + bj IR4 (IR4 diagonal)
-    double s0 = 0.0, s1 = 0.0;
+  fe += fprev;
+  sh[i] = active ? fe : 0.0;
+  wave_sync();
+  const double diag = d2ii + bj * ir4;   // L1 = D2 + bj IR4 (IR4 diagonal)
+  double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-    for (int c = 0; c < NRS; c += 2) {
-      const double c0 = (c == i) ? diag : sD2T[c * NRS + ii];
-      const double c1 = (c + 1 == i) ? diag : sD2T[(c + 1) * NRS + ii];
-      s0 = fma(c0, sh[c], s0);
-      s1 = fma(c1, sh[c + 1], s1);
-    }
-    const double L1f = s0 + s1;
-    wave_sync();
-    const double bb = (pdt * bjt) * (L1f + ir4 * bf) - bjt * (ir2 * fe);
-    sh[i] = active ? gv + bb : 0.0;
-    wave_sync();
-    y = dot_row<NRS>(Lc, sh);
-    wave_sync();
-    bf += bj * y + bjt * fe;
-    if (Pp && active) Pp[(size_t)row * nr + i] = fe;
-  } else {
-    sh[i] = active ? gv : 0.0;
-    wave_sync();
-    y = dot_row<NRS>(Lc, sh);
-    wave_sync();
-    bf += bj * y;
-    if (Pp && active) Pp[(size_t)row * nr + i] = 0.0;
+  for (int c = 0; c < NRS; c += 2) {
+    const double c0 = (c == i) ? diag : sD2T[c * NRS + ii];
+    const double c1 = (c + 1 == i) ? diag : sD2T[(c + 1) * NRS + ii];
+    s0 = fma(c0, sh[c], s0);
+    s1 = fma(c1, sh[c + 1], s1);
   }
-  if (active) x[(size_t)row * nr + i] = y;
+  const double L1f = s0 + s1;
+  wave_sync();
+  const double bb = (pdt * bjt) * (L1f + ir4 * bf) - bjt * (ir2 * fe);
+  sh[i] = active ? gv + bb : 0.0;
+  wave_sync();
+  const double y = dot_row<NRS>(Lc, sh);
+  wave_sync();
+  bf += bj * y + bjt * fe;
+  if (active) {
+    if (Pp) Pp[(size_t)row * nr + i] = fe;
+    x[(size_t)row * nr + i] = y;
+  }
   fprev = y;
   row -= 2;
 }
@@ -166,20 +167,22 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
     return;
   }
   const size_t MS = (size_t)NRP * NRP;
+  const int nsteps = start / 2 + 1;
   double La[NRS], Lb[NRS];
   double fprev = 0.0;
+  double gcur = g[(size_t)start * nr + ii];
   if (field > 0) {
     const double* Linv = (field == 1) ? a.LinvT : a.LinvS;
     const double dtf = (field == 1) ? a.dt : a.Tau * a.dt;
     double bacc = 0.0;
     load_row<NRS>(La, Linv + (size_t)(K - 1 - start) * MS, NRP, ii);
-    double gcur = g[(size_t)start * nr + ii];
     int j = start;
-    while (j >= 0) {
+    for (int s = 0; s + 1 < nsteps; s += 2) {
       nab2_step<NRS>(j, K, nr, i, active, dtf, g, x, Linv, MS, NRP, ii, La, Lb, gcur, bacc, fprev, sh);
-      if (j < 0) break;
       nab2_step<NRS>(j, K, nr, i, active, dtf, g, x, Linv, MS, NRP, ii, Lb, La, gcur, bacc, fprev, sh);
     }
+    if (nsteps & 1)
+      nab2_step<NRS>(j, K, nr, i, active, dtf, g, x, Linv, MS, NRP, ii, La, Lb, gcur, bacc, fprev, sh);
     return;
   }
   for (int e = threadIdx.x; e < NRS * NRS; e += blockDim.x) {
@@ -193,15 +196,16 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
   const double pdt = a.Pr * a.dt;
   double fe = 0.0, bf = 0.0;
   load_row<NRS>(La, a.LinvA4 + (size_t)(K - 1 - start) * MS, NRP, ii);
-  double gcur = g[(size_t)start * nr + ii];
   int row = start;
-  while (row >= 0) {
-    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, La, Lb, sD2T,
-                 d2ii, ir4, ir2, gcur, fe, bf, fprev, sh);
-    if (row < 0) break;
-    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, Lb, La, sD2T,
-                 d2ii, ir4, ir2, gcur, fe, bf, fprev, sh);
+  for (int s = 0; s + 1 < nsteps; s += 2) {
+    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, La, Lb, sD2T, d2ii,
+                 ir4, ir2, gcur, fe, bf, fprev, sh);
+    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, Lb, La, sD2T, d2ii,
+                 ir4, ir2, gcur, fe, bf, fprev, sh);
   }
+  if (nsteps & 1)
+    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, La, Lb, sD2T, d2ii,
+                 ir4, ir2, gcur, fe, bf, fprev, sh);
 }
 
 int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,

@@ -96,6 +96,7 @@ __global__ __launch_bounds__(256) void k_spectral(SpectralArgs a) {
   __shared__ double sT[kTile + 1][NRP];
   __shared__ double sS[kTile + 1][NRP];
   __shared__ double sC[5][kTile][NRP + 1];  // contraction results [c][mode][i]
+  __shared__ double sV[V_COUNT * NRP];      // radial vectors
   const int K = a.K, nr = a.nr;
   const int b = blockIdx.y;
   const int k0 = blockIdx.x * kTile;
@@ -103,18 +104,25 @@ __global__ __launch_bounds__(256) void k_spectral(SpectralArgs a) {
   const double* Xb = a.X + (size_t)b * 3 * N;
   const double* Pb = a.P + (size_t)b * N;
 
-  for (int idx = threadIdx.x; idx < (kTile + 2) * NRP; idx += blockDim.x) {
-    const int t = idx / NRP, i = idx - t * NRP;
-    const int k = k0 - 1 + t;
-    const bool ok = (k >= 0 && k < K && i < nr);
-    const size_t o = (size_t)k * nr + i;
-    sPsi[t][i] = ok ? Xb[o] : 0.0;
-    if (t >= 1) {
-      sP[t - 1][i] = ok ? Pb[o] : 0.0;
-      sT[t - 1][i] = ok ? Xb[N + o] : 0.0;
-      sS[t - 1][i] = ok ? Xb[2 * N + o] : 0.0;
+  // all tile loads of a thread are issued before any is used
+  constexpr int kLoadIters = ((kTile + 2) * NRP + 255) / 256;
+#pragma unroll
+  for (int it = 0; it < kLoadIters; ++it) {
+    const int idx = threadIdx.x + 256 * it;
+    if (idx < (kTile + 2) * NRP) {
+      const int t = idx / NRP, i = idx - t * NRP;
+      const int k = k0 - 1 + t;
+      const bool ok = (k >= 0 && k < K && i < nr);
+      const size_t o = ok ? (size_t)k * nr + i : 0;
+      sPsi[t][i] = ok ? Xb[o] : 0.0;
+      if (t >= 1) {
+        sP[t - 1][i] = ok ? Pb[o] : 0.0;
+        sT[t - 1][i] = ok ? Xb[N + o] : 0.0;
+        sS[t - 1][i] = ok ? Xb[2 * N + o] : 0.0;
+      }
     }
   }
+  for (int e = threadIdx.x; e < V_COUNT * NRP; e += blockDim.x) sV[e] = a.vecs[e];
   __syncthreads();
 
   // Five contractions (nr x nr) @ (nr x 16 modes).  Both MFMA shapes take
@@ -157,11 +165,11 @@ __global__ __launch_bounds__(256) void k_spectral(SpectralArgs a) {
   }
   __syncthreads();
 
-  const double* vIR2 = a.vecs + V_IR2 * NRP;
-  const double* vIR4 = a.vecs + V_IR4 * NRP;
-  const double* vDT0 = a.vecs + V_DT0 * NRP;
-  const double* vRSQ = a.vecs + V_RSQ * NRP;
-  const double* vGR = a.vecs + V_GR * NRP;
+  const double* vIR2 = sV + V_IR2 * NRP;
+  const double* vIR4 = sV + V_IR4 * NRP;
+  const double* vDT0 = sV + V_DT0 * NRP;
+  const double* vRSQ = sV + V_RSQ * NRP;
+  const double* vGR = sV + V_GR * NRP;
   const double Ra = a.Ra ? a.Ra[b] : 0.0;
   const double Ras = a.Ras ? a.Ras[b] : 0.0;
   const double dtpr = a.dt * a.Pr;
@@ -326,7 +334,11 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
   double* Gb = a.G + (size_t)b * 3 * N;
   if (a.have_nonlin) {
     const R* Fb = reinterpret_cast<const R*>(a.F) + (size_t)b * 4 * nK;
-    for (int idx = threadIdx.x; idx < NRP * kTile; idx += blockDim.x) {
+    constexpr int kIters = (NRP * kTile + 255) / 256;
+#pragma unroll
+    for (int it = 0; it < kIters; ++it) {
+      const int idx = threadIdx.x + 256 * it;
+      if (idx >= NRP * kTile) break;
       const int i = idx / kTile, rl = idx - i * kTile;
       const int r = k0 + rl, m = r + 1;
       const bool okm = (i < nr && m < K), okr = (i < nr && r < K);

@@ -235,9 +235,22 @@ __global__ __launch_bounds__(256) void k_nonlinear(FftShape sh, const cplx<R>* _
   __syncthreads();
   const int i = blockIdx.x;
   const int b = blockIdx.y;
-  const SlabRows<R> sr = slab_rows<R>(slabs, b, i, nr, K);
+  // Stage the seven slab rows of X in the second work buffer (idle until the
+  // first FFT): one batch of coalesced loads instead of dependent load/use
+  // chains in the synthesis loops.
+  const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
+  R* stage = reinterpret_cast<R*>(Bf);
+#pragma unroll 8
+  for (int e = threadIdx.x; e < 7 * K; e += blockDim.x) {
+    const int r = e / K;
+    stage[e] = sg.s[r][e - r * K];
+  }
+  SlabRows<R> sr;
+#pragma unroll
+  for (int r = 0; r < 7; ++r) sr.s[r] = stage + r * K;
   SlabRows<R> sd = sr;
   if (DFX) sd = slab_rows<R>(slabs2, b, i, nr, K);
+  __syncthreads();
 
   // power-of-two scales of the nine fields of X (and of dv)
   R mx[18];

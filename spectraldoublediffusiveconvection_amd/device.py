@@ -111,8 +111,8 @@ class Plan:
             if t.numel() == 1 and B > 1:
                 t = t.expand(B).contiguous()
         else:
-            arr = np.broadcast_to(np.asarray(v, dtype=np.float64), (B,))
-            t = torch.as_tensor(np.ascontiguousarray(arr), device=self.device)
+            arr = np.array(np.broadcast_to(np.asarray(v, dtype=np.float64), (B,)))
+            t = torch.as_tensor(arr, device=self.device)
         if t.numel() != B:
             raise ValueError("parameter array length must equal the batch size")
         return t
@@ -188,6 +188,14 @@ class Plan:
         out = torch.empty((X.shape[0], 4), dtype=torch.float64, device=X.device)
         _lib.check(_lib.lib().sddc_diagnostics(self._h, X.shape[0], _dev_ptr(X), _dev_ptr(out), _stream()))
         return out
+
+    def run_stage(self, X, Ra, Ra_s, stage):
+        """Launch one stage of the step on the workspace (timing/profiling)."""
+        X = self._state(X)
+        B = X.shape[0]
+        ra, ras = self._params(Ra, B), self._params(Ra_s, B)
+        _lib.check(_lib.lib().sddc_run_stage(self._h, B, int(stage), _dev_ptr(X), _dev_ptr(ra),
+                                             _dev_ptr(ras), _stream()))
 
     def graph(self, X, Ra, Ra_s, nsteps=1):
         """Capture nsteps in-place steps of the batch X into a HIP graph."""
