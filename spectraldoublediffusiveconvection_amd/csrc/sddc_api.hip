@@ -109,14 +109,6 @@ static void tile_mfma(const double* A, int nr, int NRP, T* out) {
       }
 }
 
-static void pad_square(const double* A, int nr, int NRP, double* out, bool transpose = false) {
-  for (int r = 0; r < NRP; ++r)
-    for (int c = 0; c < NRP; ++c) {
-      const double v = (r < nr && c < nr) ? A[(size_t)r * nr + c] : 0.0;
-      out[transpose ? (size_t)c * NRP + r : (size_t)r * NRP + c] = v;
-    }
-}
-
 template <typename T>
 static int upload(T** dst, const T* src, size_t n) {
   SDDC_HIP(hipMalloc(dst, n * sizeof(T)));
@@ -125,12 +117,14 @@ static int upload(T** dst, const T* src, size_t n) {
 }
 
 static void free_ws(Plan& p) {
-  void* bufs[] = {p.P, p.L, p.G, p.Xa, p.Xb, p.Pa, p.slabs, p.slabs2, p.F, p.rowval, p.params};
+  void* bufs[] = {p.P, p.L, p.G, p.Xa, p.Xb, p.Pa, p.slabs, p.slabs2, p.F, p.rowval, p.params,
+                  p.solve_scratch, p.chain_EX};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   p.P = p.L = p.G = p.Xa = p.Xb = p.Pa = nullptr;
   p.slabs = p.slabs2 = p.F = nullptr;
-  p.rowval = p.params = nullptr;
+  p.rowval = p.params = p.solve_scratch = p.chain_EX = nullptr;
+  p.ex_batch = 0;
   p.ws_bytes = 0;
   p.max_batch = 0;
 }
@@ -159,6 +153,12 @@ static int alloc_ws(Plan& p, int B) {
   if ((rc = A(&p.F, (size_t)B * 4 * N * rs))) return rc;
   if ((rc = A((void**)&p.rowval, (size_t)B * p.nr * 8))) return rc;
   if ((rc = A((void**)&p.params, (size_t)2 * B * 8))) return rc;
+  // sink of padded instances and rows: X at [0, N), P at [N, 2N), lanes from 2N
+  if ((rc = A((void**)&p.solve_scratch, (4 * N + 256) * 8))) return rc;
+  // segment exit states of the solve, for B rounded up to whole workgroups
+  const int exb = ((B + 3) / 4) * 4;
+  if ((rc = A((void**)&p.chain_EX, (size_t)exb * 6 * p.nseg * 2 * p.NRP * 8))) return rc;
+  p.ex_batch = exb;
   p.ws_bytes = total;
   p.max_batch = B;
   return SDDC_OK;
@@ -220,7 +220,7 @@ int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric, in
   (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p.device);
   if ((size_t)nonlinear_lds_bytes(p, precision, 0) > (size_t)(lds_max > 0 ? lds_max : 65536))
     return bail(fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel"));
-  const int nr = p.nr, NRP = p.NRP, K = p.K, Np1 = p.N_r + 1;
+  const int nr = p.nr, NRP = p.NRP, Np1 = p.N_r + 1;
   if ((rc = upload(&p.Dfull, ops->D, (size_t)Np1 * Np1))) return bail(rc);
   if ((rc = upload(&p.R, ops->R, (size_t)Np1))) return bail(rc);
   const size_t T = (size_t)NRP * NRP;
@@ -245,18 +245,8 @@ int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric, in
     v[V_IR * NRP + i] = 1. / ops->R[i + 1];
   }
   if ((rc = upload(&p.vecs, v.data(), v.size()))) return bail(rc);
-  std::vector<double> sq(T);
-  pad_square(ops->D2_A4, nr, NRP, sq.data());
-  if ((rc = upload(&p.D2A4, sq.data(), T))) return bail(rc);
-  std::vector<double> stack((size_t)K * T);
-  const double* inv[3] = {ops->Linv_A4, ops->Linv_T, ops->Linv_S};
-  double** dst[3] = {&p.LinvA4, &p.LinvT, &p.LinvS};
-  for (int f = 0; f < 3; ++f) {
-    // stored transposed: the solve kernel reads block columns
-    for (int jj = 0; jj < K; ++jj)
-      pad_square(inv[f] + (size_t)jj * nr * nr, nr, NRP, stack.data() + jj * T, true);
-    if ((rc = upload(dst[f], stack.data(), stack.size()))) return bail(rc);
-  }
+  p.NRS = ((nr + 7) / 8) * 8;
+  if ((rc = build_chain(p, ops, max_batch > 0 ? max_batch : 1))) return bail(rc);
   if ((rc = build_tables(p.tabM, p.M))) return bail(rc);
   if ((rc = build_tables(p.tabKE, 3 * p.K))) return bail(rc);
   if ((rc = alloc_ws(p, max_batch > 0 ? max_batch : 1))) return bail(rc);
@@ -274,9 +264,10 @@ int sddc_plan_destroy(sddc_plan* plan) {
   if (!plan) return SDDC_OK;
   Plan& p = plan->p;
   free_ws(p);
-  void* bufs[] = {p.Dfull, p.R, p.mfma64, p.mfma32, p.vecs, p.D2A4, p.LinvA4, p.LinvT, p.LinvS};
+  void* bufs[] = {p.Dfull, p.R, p.mfma64, p.mfma32, p.vecs};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  free_chain(p);
   free_tables(p.tabM);
   free_tables(p.tabKE);
   delete plan;

@@ -92,6 +92,15 @@ __device__ __forceinline__ void dft5(cplx<R>* v, bool inv) {
   v[3] = {r2.x - sg * q2.y, r2.y + sg * q2.x};
 }
 
+// Quotient w/d for 0 <= w < 2^20 from a float reciprocal of d, corrected to
+// exact (integer division proper costs dozens of instructions per use).
+__device__ __forceinline__ int qdiv(int w, int d, float rd) {
+  int q = (int)((float)w * rd);
+  const int r = w - q * d;
+  q += (r >= d) - (r < 0);
+  return q;
+}
+
 // One Stockham autosort pass of radix RADIX over nfft arrays of length L,
 // out of place.  Ns is the product of the radices already applied.
 template <typename R, int RADIX>
@@ -99,12 +108,14 @@ __device__ void fft_pass(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ o
                          int nfft, int Ns, const cplx<R>* __restrict__ tw, bool inv) {
   const int nb = L / RADIX;
   const int st = L / (Ns * RADIX);
+  const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)Ns;
   for (int w = threadIdx.x; w < nfft * nb; w += blockDim.x) {
-    const int f = w / nb;
+    const int f = qdiv(w, nb, rnb);
     const int j = w - f * nb;
     const cplx<R>* src = in + f * L;
     cplx<R>* dst = out + f * L;
-    const int k = j % Ns;
+    const int jq = qdiv(j, Ns, rns);
+    const int k = j - jq * Ns;
     cplx<R> v[RADIX];
     v[0] = src[j];
 #pragma unroll
@@ -113,7 +124,7 @@ __device__ void fft_pass(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ o
     if (RADIX == 3) dft3(v, inv);
     if (RADIX == 4) dft4(v, inv);
     if (RADIX == 5) dft5(v, inv);
-    const int base = (j / Ns) * Ns * RADIX + k;
+    const int base = jq * Ns * RADIX + k;
 #pragma unroll
     for (int s = 0; s < RADIX; ++s) dst[base + s * Ns] = v[s];
   }
@@ -203,47 +214,55 @@ __device__ __forceinline__ R syn_coef(const R* __restrict__ row, int K, int L, i
 // error relative to the larger of the two.  Each series is therefore scaled
 // by a power of two (exact) to unit magnitude before packing, and unscaled
 // afterwards.
+extern "C" __device__ int __ockl_wfred_max_i32(int);
+// Binary exponent e with |x| < 2^e (frexp); zero maps to a large negative value.
 template <typename R>
-__device__ __forceinline__ R pow2_scale(R mx) {
-  if (!(mx > (R)0) || !(mx < (R)3.0e38)) return (R)1;
+__device__ __forceinline__ int bexp(R x) {
   int e;
-  (void)frexp(mx, &e);
+  (void)frexp(x, &e);
+  return x == (R)0 ? -1000 : e;
+}
+
+// Power of two 2^-e bringing magnitudes below 2^e into [0.5, 1).
+template <typename R>
+__device__ __forceinline__ R pow2_scale_e(int e) {
+  if (e < -900 || e > 900) return (R)1;
   return ldexp((R)1, -e);
 }
 
-// Block-wide maximum of NV values (256 threads = 4 wavefronts); red holds
-// 4*NV entries of LDS.  Ends with a barrier; every thread gets the result.
-template <int NV, typename R>
-__device__ __forceinline__ void block_max(R (&v)[NV], R* red) {
+// Block-wide maximum of NV exponents: DPP wavefront reduction, then one LDS
+// word per wavefront.  red needs (blockDim/64)*NV ints.  Ends with a barrier.
+template <int NV>
+__device__ __forceinline__ void block_max_exp(int (&e)[NV], int* red) {
 #pragma unroll
-  for (int k = 0; k < NV; ++k)
-    for (int off = 32; off > 0; off >>= 1) v[k] = fmax(v[k], __shfl_xor(v[k], off, 64));
+  for (int k = 0; k < NV; ++k) e[k] = __ockl_wfred_max_i32(e[k]);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   if (l == 0)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) red[w * NV + k] = v[k];
+    for (int k = 0; k < NV; ++k) red[w * NV + k] = e[k];
   __syncthreads();
   const int nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    R m = (R)0;
-    for (int q = 0; q < nw; ++q) m = fmax(m, red[q * NV + k]);
-    v[k] = m;
+    int m = -1000;
+    for (int q = 0; q < nw; ++q) m = max(m, red[q * NV + k]);
+    e[k] = m;
   }
   __syncthreads();
 }
 
-// Largest |a''_k| of a series row (see syn_coef).
+// Largest exponent of |a''_k| over a series row held by this thread's part
+// of the row (see syn_coef); combine with block_max_exp.
 template <typename R>
-__device__ __forceinline__ R series_absmax(const R* __restrict__ row, int K, bool is_sin,
-                                           int mult) {
-  R m = (R)0;
+__device__ __forceinline__ int series_maxexp(const R* __restrict__ row, int K, bool is_sin,
+                                             int mult) {
+  int m = -1000;
   if (row == nullptr) return m;
   for (int k = threadIdx.x; k < K; k += blockDim.x) {
     if (is_sin && k == 0) continue;
     R c = row[k];
     if (mult) c = (R)k * c;
-    m = fmax(m, fabs(c));
+    m = max(m, bexp(c));
   }
   return m;
 }

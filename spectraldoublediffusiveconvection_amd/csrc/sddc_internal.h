@@ -78,10 +78,13 @@ struct Plan {
   double* mfma64 = nullptr;    // MFMA-tiled Dr | D2r2 | D2s   (3 * NRP*NRP)
   float* mfma32 = nullptr;     // MFMA-tiled Dr | D2r2          (2 * NRP*NRP)
   double* vecs = nullptr;      // 8 vectors of length NRP (see VecId)
-  double* D2A4 = nullptr;      // NRP x NRP row-major, zero padded
-  double* LinvA4 = nullptr;    // K x NRP x NRP, index jj, each block transposed
-  double* LinvT = nullptr;
-  double* LinvS = nullptr;
+  // implicit solves: quad-packed operator data (layout in sddc_solve.hip)
+  int NRS = 0;                 // nr rounded up to a multiple of 8
+  int seg_m = 0, nseg = 0;     // rows per chain segment, segments per chain
+  double* chain_L[3] = {nullptr, nullptr, nullptr};   // block inverses, K x NRP x NRS
+  double* chain_D2 = nullptr;                         // D2 of the A4 solve, NRP x NRS
+  double* chain_Q[3] = {nullptr, nullptr, nullptr};   // row corrections (nseg > 1)
+  double* chain_E[3] = {nullptr, nullptr, nullptr};   // segment exit maps (nseg > 1)
   TableSet tabM, tabKE;
   // workspace (device), sized for max_batch instances
   double* P = nullptr;
@@ -95,6 +98,10 @@ struct Plan {
   void* F = nullptr;
   double* rowval = nullptr;
   double* params = nullptr;    // 2*max_batch: Ra, Ra_s staging
+  double* solve_scratch = nullptr;   // 4N + 256 doubles, sink of padded solve stores
+  double* chain_EX = nullptr;        // zero-entry segment exit states
+  int ex_batch = 0;                  // instances chain_EX holds
+  int tune_solve_nbi = 0;            // instances per solve workgroup, 0 = automatic
   size_t ws_bytes = 0;
 };
 
@@ -128,6 +135,8 @@ int launch_rhs(const Plan& p, int B, const void* F, const double* L, int have_no
                double* G, hipStream_t s);
 int launch_solve(const Plan& p, int B, const double* G, double* Xout, double* Pout,
                  int fields_mask, int mask_out, hipStream_t s);
+int build_chain(Plan& p, const sddc_operators* ops, int max_batch);
+void free_chain(Plan& p);
 int launch_mask(const Plan& p, int B, const double* X, double* out, hipStream_t s);
 int launch_transform(const TableSet& t, int kind, int rows, int len_in, int n_out,
                      const double* in, double* out, hipStream_t s);

@@ -2,253 +2,629 @@
 // theta modes with pre-inverted per-mode radial blocks.
 //   A4_BSub_TSTEP_V2   (Matrix_Operators.py:1115-1194)  field 0 (psi)
 //   NAB2_BSub_TSTEP_V2 (Matrix_Operators.py:1033-1086)  fields 1 (T), 2 (S)
-// The recurrence over modes is sequential; the two parity chains of each
-// field are independent.  One wavefront handles one (field, chain, instance);
-// lane i owns radial row i: its row of the next block inverse is fetched into
-// registers one mode ahead, and the right-hand side vector is broadcast
-// through LDS.  The block inverses are stored transposed (column c of a block
-// is contiguous), so each load instruction of a row fetch is one coalesced
-// read across the wavefront.
+//
+// Each field has two independent chains (the two row parities).  A chain is a
+// linear recurrence over its rows, high to low: a state vector (NAB2: the
+// running sum b, nr values; A4: the running sums f_e and bf_e, 2 nr values)
+// is updated after every row, and the row's solution is Linv_row applied to
+// g_row plus terms in the state.  The chain is cut into segments of m rows.
+// The recurrence is affine in the state, so
+//   * the solution of a row is the one computed with the segment's entry state
+//     set to zero, plus Q_row times the true entry state;
+//   * a segment's exit state is its zero-entry exit plus E_seg times its entry
+//     state.
+// Q and E depend on the operators only and are built once per plan.  A solve
+// is two launches:
+//   k_chain_local  every segment of every chain at once, zero entry state;
+//   k_chain_join   entry states by the short recurrence over segments, then
+//                  the Q corrections of the rows, which are independent.
+// The sequential depth of a chain drops from K/2 rows to m + K/(2m).
+//
+// Work layout: a workgroup holds NRP/16 wavefronts; wavefront w owns radial
+// rows 16w..16w+15 with four lanes per row (a quad): lane l has row
+// 16w + l/4 and the columns c = l (mod 4) of that row.  Matrices are stored
+// quad-packed, so that every 16-byte load instruction of a wavefront reads
+// 1 KiB of consecutive memory: element (r, c) of an NRP x NC matrix sits at
+//   ((w*(NC/8) + q)*64 + l)*2 + h,   r = 16w + l/4,  c = l%4 + 8q + 4h.
+// Rows r >= nr are padding: their matrix rows and columns are zero and their
+// vector entries stay finite (loads are clamped to row nr-1), so they never
+// reach a real row and are written without selects or branches.
+// A matrix-vector product is one partial dot product per lane (NC/4 terms)
+// and a quad reduction (two DPP exchanges, which leave the same sum in all
+// four lanes).  The vector is broadcast through LDS, stored as [c%4][c/4] so
+// each lane reads its NC/4 entries contiguously.  Operator data of the next
+// kDepth rows is loaded into registers ahead of use.
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
 #include "sddc_internal.h"
 
 namespace sddc {
 
-struct SolveArgs {
-  const double* G;
-  double* X;
-  double* Pout;
-  const double* LinvA4;
-  const double* LinvT;
-  const double* LinvS;
-  const double* D2A4;
+constexpr int kDepth = 4;    // chain rows of operator data in flight per lane
+constexpr int kDepthE = 2;   // segment exit maps in flight per lane (join)
+
+struct ChainArgs {
+  const double* G;          // right-hand sides, field f of instance b at b*gstride+goff[f]
+  double* X;                // solutions, b*xstride + xoff[f]
+  double* P;                // psi suffix sums [B][N] or null
+  const double* Lq[3];      // packed block inverses per field, block index jj = K-1-row
+  const double* D2q;        // packed D2 of the A4 solve
+  const double* Qq[3];      // packed corrections, index jj
+  const double* Eq[3];      // packed segment exit maps, index chain*nseg + seg
   const double* vecs;
+  double* EX;               // zero-entry exit states [b][3][2][nseg][2*NRP]
+  double* scratch;          // 2N + 256 doubles: sink of stores of padded instances/rows
   long long gstride, xstride;
   long long goff[3], xoff[3];
-  int K, nr, NRP, symmetric, fields_mask;
-  double dt, Pr, Tau;
+  int K, nr, B, m, nseg, symmetric, fields_mask;
+  double dtf[3];            // Pr*dt, dt, Tau*dt
 };
 
-// The solve workgroup is one wavefront: LDS writes of a wavefront are seen by
-// its later LDS reads, so ordering needs only a compiler barrier, not
-// __syncthreads() (whose s_waitcnt vmcnt(0) would also drain the operator
-// rows prefetched for the next mode).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+template <int NRS>
+struct CShape {
+  static constexpr int NRP = ((NRS + 15) / 16) * 16;
+  static constexpr int WR = NRP / 16;     // wavefronts per workgroup
+  static constexpr int NQ = NRS / 8;      // 16-byte loads per lane of an NRP x NRS block
+  static constexpr int CPL = NRS / 4;     // entries of an NRS vector per lane
+  static constexpr int BLK = NRP * NRS;   // doubles per packed NRP x NRS block
+};
+
+// Position of vector entry c in the LDS broadcast layout of a vector with
+// 4*cpl entries.
+__device__ __forceinline__ int vpos(int c, int cpl) { return (c & 3) * cpl + (c >> 2); }
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
 
-template <int NRS>
-__device__ __forceinline__ void load_row(double (&Lr)[NRS], const double* __restrict__ blockT,
-                                         int NRP, int ii) {
+// Sum over the four lanes of a quad; every lane gets the same value.
+__device__ __forceinline__ double quad_sum(double v) {
+  v += dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  return v;
+}
+
+// Workgroup barrier for the LDS broadcast: the wavefront's own LDS writes are
+// complete (lgkmcnt(0)) before s_barrier.  No vmcnt wait, so loads in flight
+// stay in flight.  Signal fences keep the compiler from moving LDS accesses
+// across it.
+__device__ __forceinline__ void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // vmcnt 63 (none), expcnt 7 (none), lgkmcnt 0
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+template <int NQ>
+__device__ __forceinline__ void load_blk(double2 (&a)[NQ], const double* p) {
 #pragma unroll
-  for (int c = 0; c < NRS; ++c) Lr[c] = blockT[(size_t)c * NRP + ii];
+  for (int q = 0; q < NQ; ++q) a[q] = *reinterpret_cast<const double2*>(p + q * 128);
 }
 
-template <int NRS>
-__device__ __forceinline__ double dot_row(const double (&Lr)[NRS], const double* shv) {
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-#pragma unroll
-  for (int c = 0; c < NRS; c += 4) {
-    s0 = fma(Lr[c], shv[c], s0);
-    s1 = fma(Lr[c + 1], shv[c + 1], s1);
-    s2 = fma(Lr[c + 2], shv[c + 2], s2);
-    s3 = fma(Lr[c + 3], shv[c + 3], s3);
-  }
-  return (s0 + s1) + (s2 + s3);
-}
-
-// The steps below have no data-dependent branches: the prefetches of the next
-// mode always issue (indices clamped at the last mode) and the first-row cases
-// of the reference are selects.  With straight-line loop bodies the compiler
-// can wait only for the loads of the previous step instead of draining all
-// outstanding loads (s_waitcnt vmcnt(0)) every step.
-
-// One NAB2 step (Matrix_Operators.py:1061-1083): b += 2 dt (j+2) f_{j+2};
-// f_j = Linv_{K-1-j} (g_j - c_j b), c_0 = 1/2.  At the first row of a chain
-// fprev = 0, so the unconditional update adds a signed zero (exact).
-template <int NRS>
-__device__ __forceinline__ void nab2_step(int& j, int K, int nr, int i, bool active, double dtf,
-                                          const double* __restrict__ g, double* __restrict__ x,
-                                          const double* __restrict__ Linv, size_t MS, int NRP,
-                                          int ii, double (&Lc)[NRS], double (&Ln)[NRS],
-                                          double& gcur, double& bacc, double& fprev, double* sh) {
-  const int jj = K - 1 - j;
-  const int jn = jj + 2 < K ? jj + 2 : K - 1;
-  load_row<NRS>(Ln, Linv + (size_t)jn * MS, NRP, ii);
-  const double gv = gcur;
-  gcur = g[(size_t)(j >= 2 ? j - 2 : 0) * nr + ii];
-  double bk = -2.0 * (j + 2.0);
-  bk = -dtf * bk;
-  bacc += bk * fprev;
-  const double rhs = (j == 0) ? gv - 0.5 * bacc : gv - bacc;
-  sh[i] = active ? rhs : 0.0;
-  wave_sync();
-  const double y = dot_row<NRS>(Lc, sh);
-  wave_sync();
-  if (active) x[(size_t)j * nr + i] = y;
-  fprev = y;
-  j -= 2;
-}
-
-// One A4 step (Matrix_Operators.py:1141-1164), dt = Pr*dt:
-//   f_e += f_{row+2};  b = dt bjt (L1 f_e + IR4 bf_e) - bjt IR2 f_e
-//   f_row = Linv (g_row + b);  bf_e += bj f_row + bjt f_e
-// At the first row of a chain f_e = bf_e = fprev = 0, so b is a signed zero
-// and the updates reduce exactly to the reference's first-row branch.
-template <int NRS>
-__device__ __forceinline__ void a4_step(int& row, int K, int nr, int i, bool active, double pdt,
-                                        const double* __restrict__ g, double* __restrict__ x,
-                                        double* __restrict__ Pp, const double* __restrict__ Linv,
-                                        size_t MS, int NRP, int ii, double (&Lc)[NRS],
-                                        double (&Ln)[NRS], const double* sD2T, double d2ii,
-                                        double ir4, double ir2, double& gcur, double& fe,
-                                        double& bf, double& fprev, double* sh) {
-  const int jj = K - 1 - row;
-  const int jn = jj + 2 < K ? jj + 2 : K - 1;
-  load_row<NRS>(Ln, Linv + (size_t)jn * MS, NRP, ii);
-  const double gv = gcur;
-  gcur = g[(size_t)(row >= 2 ? row - 2 : 0) * nr + ii];
-  const double j = (double)(K - jj);
-  const double bj = -j * (j + 1.0);
-  const double bjt = -2.0 * j;
-  fe += fprev;
-  sh[i] = active ? fe : 0.0;
-  wave_sync();
-  const double diag = d2ii + bj * ir4;   // L1 = D2 + bj IR4 (IR4 diagonal)
+// Partial dot product of the lane's packed matrix entries with its vector
+// entries x[0 .. 2NQ) (column l%4 + 8q + 4h pairs with x[2q + h]).
+template <int NQ>
+__device__ __forceinline__ double pdot(const double2 (&a)[NQ], const double* x) {
   double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-  for (int c = 0; c < NRS; c += 2) {
-    const double c0 = (c == i) ? diag : sD2T[c * NRS + ii];
-    const double c1 = (c + 1 == i) ? diag : sD2T[(c + 1) * NRS + ii];
-    s0 = fma(c0, sh[c], s0);
-    s1 = fma(c1, sh[c + 1], s1);
+  for (int q = 0; q < NQ; ++q) {
+    const double2 v = *reinterpret_cast<const double2*>(x + 2 * q);
+    s0 = fma(a[q].x, v.x, s0);
+    s1 = fma(a[q].y, v.y, s1);
   }
-  const double L1f = s0 + s1;
-  wave_sync();
-  const double bb = (pdt * bjt) * (L1f + ir4 * bf) - bjt * (ir2 * fe);
-  sh[i] = active ? gv + bb : 0.0;
-  wave_sync();
-  const double y = dot_row<NRS>(Lc, sh);
-  wave_sync();
-  bf += bj * y + bjt * fe;
-  if (active) {
-    if (Pp) Pp[(size_t)row * nr + i] = fe;
-    x[(size_t)row * nr + i] = y;
-  }
-  fprev = y;
-  row -= 2;
+  return s0 + s1;
 }
 
-template <int NRS>
-__global__ __launch_bounds__(64) void k_solve(SolveArgs a) {
-  __shared__ double sh[64];
-  __shared__ double sD2T[NRS * NRS];   // D2 of the A4 solve, transposed
-  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1;
+template <int NQ>
+__device__ __forceinline__ double pdot_reg(const double2 (&a)[NQ], const double (&x)[2 * NQ]) {
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    s0 = fma(a[q].x, x[2 * q], s0);
+    s1 = fma(a[q].y, x[2 * q + 1], s1);
+  }
+  return s0 + s1;
+}
+
+template <int NB>
+struct Inst {
+  const double* g[NB];
+  double* x[NB];
+  double* p[NB];
+  double* ex[NB];   // exit states of segment 0 of this (field, chain)
+};
+
+template <int NB>
+__device__ __forceinline__ Inst<NB> instances(const ChainArgs& a, int field, int chain) {
+  Inst<NB> in;
+  const size_t N = (size_t)a.K * a.nr;
+  const size_t exs = (size_t)a.nseg * 2 * (((a.nr + 15) / 16) * 16);
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    const int b = blockIdx.z * NB + t;
+    const bool valid = b < a.B;
+    const int bc = valid ? b : a.B - 1;
+    in.g[t] = a.G + (size_t)bc * a.gstride + a.goff[field];
+    // instances past the batch write into scratch, so every wavefront runs
+    // the same instructions
+    in.x[t] = valid ? a.X + (size_t)b * a.xstride + a.xoff[field] : a.scratch;
+    in.p[t] = (a.P && valid) ? a.P + (size_t)b * N : a.scratch + N;
+    in.ex[t] = a.EX + (((size_t)b * 3 + field) * 2 + chain) * exs;
+  }
+  return in;
+}
+
+// ---------------------------------------------------------------------------
+// k_chain_local: one segment with zero entry state.
+// ---------------------------------------------------------------------------
+template <int NRS, int NB>
+__device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int start, int k0,
+                                           int k1, const Inst<NB>& in,
+                                           double (*xs)[NB][NRS + 64], double* exit_seg) {
+  using S = CShape<NRS>;
+  const int K = a.K, nr = a.nr;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r = 16 * w + (lane >> 2), kq = lane & 3;
+  const bool rv = r < nr;
+  const int rr = rv ? r : nr - 1;
+  const bool writer = kq == 0 && r < NRS;
+  const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
+  const double dtf = a.dtf[field];
+  const double* lb = a.Lq[field] + w * S::NQ * 128 + 2 * lane;
+  double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
+  double2 blk[kDepth][S::NQ];
+  double gv[kDepth][NB];
+  auto fetch = [&](int k, double2 (&bq)[S::NQ], double (&g)[NB]) {
+    const int j = start - 2 * (k < k1 ? k : k1 - 1);
+    load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
+  };
+#pragma unroll
+  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  double b[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) b[t] = 0.0;
+  for (int kb = k0; kb < k1; kb += kDepth) {
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) {
+      const int k = kb + d;
+      const bool live = k < k1;
+      const int j = start - 2 * (live ? k : k1 - 1);
+      // NAB2 (Matrix_Operators.py:1061-1083): f_j = Linv (g_j - c_j b),
+      // c_0 = 1/2; then b += 2 dt j f_j for row j-2.
+      const double cj = (j == 0) ? 0.5 : 1.0;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double rhs = gv[d][t] - cj * b[t];
+        xs[d & 1][t][wp] = rhs;
+      }
+      lds_barrier();
+      const double alpha = 2.0 * dtf * j;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[d & 1][t][kq * S::CPL]));
+        double* dst = (live && rv) ? in.x[t] + (size_t)j * nr + r : sink;
+        *dst = f;
+        b[t] = live ? b[t] + alpha * f : b[t];
+      }
+      fetch(k + kDepth, blk[d], gv[d]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    double* dst = rv ? in.ex[t] + (exit_seg - a.EX) + r : sink;
+    *dst = b[t];
+  }
+}
+
+template <int NRS, int NB>
+__device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, int k1,
+                                         const Inst<NB>& in, double (*xs)[NB][NRS + 64],
+                                         double* exit_seg) {
+  using S = CShape<NRS>;
+  const int K = a.K, nr = a.nr;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r = 16 * w + (lane >> 2), kq = lane & 3;
+  const bool rv = r < nr;
+  const int rr = rv ? r : nr - 1;
+  const bool writer = kq == 0 && r < NRS;
+  const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
+  const double dta = a.dtf[0];
+  const double ir4 = a.vecs[V_IR4A4 * S::NRP + rr];
+  const double ir2 = a.vecs[V_IR2A4 * S::NRP + rr];
+  double2 d2[S::NQ];
+  load_blk<S::NQ>(d2, a.D2q + w * S::NQ * 128 + 2 * lane);
+  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lane;
+  double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
+  double2 blk[kDepth][S::NQ];
+  double gv[kDepth][NB];
+  auto fetch = [&](int k, double2 (&bq)[S::NQ], double (&g)[NB]) {
+    const int j = start - 2 * (k < k1 ? k : k1 - 1);
+    load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
+  };
+#pragma unroll
+  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  double fe[NB], bfe[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) fe[t] = bfe[t] = 0.0;
+  for (int kb = k0; kb < k1; kb += kDepth) {
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) {
+      const int k = kb + d;
+      const bool live = k < k1;
+      const int row = start - 2 * (live ? k : k1 - 1);
+      // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt, j = row + 1:
+      // b = dt bjt (L1 f_e + IR4 bf_e) - bjt IR2 f_e with L1 = D2 + bj IR4;
+      // f = Linv (g + b); bf_e += bj f + bjt f_e; f_e += f.  f_e is also the
+      // same-parity suffix sum P of the row.
+      const double jp = (double)(row + 1);
+      const double bj = -jp * (jp + 1.0);
+      const double bjt = -2.0 * jp;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        double* pdst = (live && rv) ? in.p[t] + (size_t)row * nr + r : sink;
+        *pdst = fe[t];
+        xs[0][t][wp] = fe[t];
+      }
+      lds_barrier();
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double d2fe = quad_sum(pdot<S::NQ>(d2, &xs[0][t][kq * S::CPL]));
+        const double l1fe = d2fe + bj * (ir4 * fe[t]);
+        const double bb = (dta * bjt) * (l1fe + ir4 * bfe[t]) - bjt * (ir2 * fe[t]);
+        xs[1][t][wp] = gv[d][t] + bb;
+      }
+      lds_barrier();
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[1][t][kq * S::CPL]));
+        double* dst = (live && rv) ? in.x[t] + (size_t)row * nr + r : sink;
+        *dst = f;
+        const double nbfe = bfe[t] + bj * f + bjt * fe[t];
+        bfe[t] = live ? nbfe : bfe[t];
+        fe[t] = live ? fe[t] + f : fe[t];
+      }
+      fetch(k + kDepth, blk[d], gv[d]);
+    }
+  }
+  const size_t off = exit_seg - a.EX;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    double* df = rv ? in.ex[t] + off + r : sink;
+    double* db = rv ? in.ex[t] + off + S::NRP + r : sink;
+    *df = fe[t];
+    *db = bfe[t];
+  }
+}
+
+template <int NRS, int NB>
+__global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs a) {
+  using S = CShape<NRS>;
+  __shared__ __align__(16) double xs[2][NB][NRS + 64];
+  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y;
   if (!((a.fields_mask >> field) & 1)) return;
-  const int b = blockIdx.y, i = threadIdx.x;
-  const int K = a.K, nr = a.nr, NRP = a.NRP;
-  const bool active = i < nr;
-  const int ii = active ? i : 0;
-  const double* g = a.G + (size_t)b * a.gstride + a.goff[field];
-  double* x = a.X + (size_t)b * a.xstride + a.xoff[field];
-  double* Pp = (field == 0 && a.Pout) ? a.Pout + (size_t)b * K * nr : nullptr;
-  const int start = (chain == 0) ? K - 1 : K - 2;
-  if (start < 0) return;
+  const int start = a.K - 1 - chain;
+  const int n = start / 2 + 1;
+  const int k0 = seg * a.m;
+  if (k0 >= n) return;
+  const int k1 = k0 + a.m < n ? k0 + a.m : n;
+  const Inst<NB> in = instances<NB>(a, field, chain);
   // Eq_SYM keeps odd psi rows (A4 chain from K-1) and even T/S rows (NAB2
-  // chain from K-2); the other chain is not solved (Matrix_Operators.py:1054,1166).
-  const bool skip = a.symmetric && (field == 0 ? chain == 1 : chain == 0);
-  if (skip) {
-    for (int r = start; r >= 0; r -= 2)
-      if (active) {
-        x[(size_t)r * nr + i] = 0.0;
-        if (Pp) Pp[(size_t)r * nr + i] = 0.0;
+  // chain from K-2); the other chain is zero (Matrix_Operators.py:1054,1166).
+  if (a.symmetric && (field == 0 ? chain == 1 : chain == 0)) {
+    const int lane = threadIdx.x & 63;
+    const int r = 16 * (threadIdx.x >> 6) + (lane >> 2);
+    if ((lane & 3) == 0 && r < a.nr)
+      for (int k = k0; k < k1; ++k) {
+        const size_t o = (size_t)(start - 2 * k) * a.nr + r;
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+          in.x[t][o] = 0.0;
+          if (field == 0) in.p[t][o] = 0.0;
+        }
       }
     return;
   }
-  const size_t MS = (size_t)NRP * NRP;
-  const int nsteps = start / 2 + 1;
-  double La[NRS], Lb[NRS];
-  double fprev = 0.0;
-  double gcur = g[(size_t)start * nr + ii];
-  if (field > 0) {
-    const double* Linv = (field == 1) ? a.LinvT : a.LinvS;
-    const double dtf = (field == 1) ? a.dt : a.Tau * a.dt;
-    double bacc = 0.0;
-    load_row<NRS>(La, Linv + (size_t)(K - 1 - start) * MS, NRP, ii);
-    int j = start;
-    for (int s = 0; s + 1 < nsteps; s += 2) {
-      nab2_step<NRS>(j, K, nr, i, active, dtf, g, x, Linv, MS, NRP, ii, La, Lb, gcur, bacc, fprev, sh);
-      nab2_step<NRS>(j, K, nr, i, active, dtf, g, x, Linv, MS, NRP, ii, Lb, La, gcur, bacc, fprev, sh);
+  double* exit_seg = a.EX + (size_t)seg * 2 * S::NRP;   // offset within an instance's (field, chain)
+  if (field == 0)
+    local_a4<NRS, NB>(a, start, k0, k1, in, xs, exit_seg);
+  else
+    local_nab2<NRS, NB>(a, field, start, k0, k1, in, xs, exit_seg);
+}
+
+// ---------------------------------------------------------------------------
+// k_chain_join: entry state of a segment, then the row corrections.
+// ---------------------------------------------------------------------------
+template <int NRS, int NB>
+__device__ __forceinline__ void join_nab2(const ChainArgs& a, int field, int chain, int start,
+                                          int seg, int k0, int k1, const Inst<NB>& in,
+                                          double (*xs)[NB][2 * NRS + 64]) {
+  using S = CShape<NRS>;
+  const int K = a.K, nr = a.nr;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r = 16 * w + (lane >> 2), kq = lane & 3;
+  const bool rv = r < nr;
+  const int rr = rv ? r : nr - 1;
+  const bool writer = kq == 0 && r < NRS;
+  const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
+  double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
+  // entry state: s_0 = 0, s_{u+1} = ex_u + E_u s_u for u < seg
+  const double* eb = a.Eq[field] + (size_t)chain * a.nseg * S::BLK + w * S::NQ * 128 + 2 * lane;
+  double2 e[kDepthE][S::NQ];
+  double exv[kDepthE][NB];
+  auto fetch_e = [&](int u, double2 (&eq)[S::NQ], double (&x)[NB]) {
+    const int uc = u < seg ? u : seg - 1;
+    load_blk<S::NQ>(eq, eb + (size_t)uc * S::BLK);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) x[t] = in.ex[t][(size_t)uc * 2 * S::NRP + rr];
+  };
+#pragma unroll
+  for (int d = 0; d < kDepthE; ++d) fetch_e(d, e[d], exv[d]);
+  double sv[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) sv[t] = 0.0;
+  for (int ub = 0; ub < seg; ub += kDepthE) {
+#pragma unroll
+    for (int d = 0; d < kDepthE; ++d) {
+      const bool live = ub + d < seg;
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+        xs[d & 1][t][wp] = sv[t];
+      lds_barrier();
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double nv = exv[d][t] + quad_sum(pdot<S::NQ>(e[d], &xs[d & 1][t][kq * S::CPL]));
+        sv[t] = live ? nv : sv[t];
+      }
+      fetch_e(ub + d + kDepthE, e[d], exv[d]);
     }
-    if (nsteps & 1)
-      nab2_step<NRS>(j, K, nr, i, active, dtf, g, x, Linv, MS, NRP, ii, La, Lb, gcur, bacc, fprev, sh);
-    return;
   }
-  for (int e = threadIdx.x; e < NRS * NRS; e += blockDim.x) {
-    const int r = e / NRS, c = e - r * NRS;
-    sD2T[c * NRS + r] = a.D2A4[(size_t)r * NRP + c];
+  // broadcast the entry state once (buffer 0: the prefix loop ran an even
+  // number of exchanges)
+#pragma unroll
+  for (int t = 0; t < NB; ++t)
+    xs[0][t][wp] = sv[t];
+  lds_barrier();
+  double xl[NB][2 * S::NQ];
+#pragma unroll
+  for (int t = 0; t < NB; ++t)
+#pragma unroll
+    for (int q = 0; q < 2 * S::NQ; ++q) xl[t][q] = xs[0][t][kq * S::CPL + q];
+  // corrections f_row += Q_row s (independent rows)
+  const double* qb = a.Qq[field] + w * S::NQ * 128 + 2 * lane;
+  double2 qv[kDepth][S::NQ];
+  double fl[kDepth][NB];
+  auto fetch_q = [&](int k, double2 (&qq)[S::NQ], double (&f)[NB]) {
+    const int j = start - 2 * (k < k1 ? k : k1 - 1);
+    load_blk<S::NQ>(qq, qb + (size_t)(K - 1 - j) * S::BLK);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) f[t] = in.x[t][(size_t)j * nr + rr];
+  };
+#pragma unroll
+  for (int d = 0; d < kDepth; ++d) fetch_q(k0 + d, qv[d], fl[d]);
+  for (int kb = k0; kb < k1; kb += kDepth) {
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) {
+      const int k = kb + d;
+      const bool live = k < k1;
+      const int j = start - 2 * (live ? k : k1 - 1);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double f = fl[d][t] + quad_sum(pdot_reg<S::NQ>(qv[d], xl[t]));
+        double* dst = (live && rv) ? in.x[t] + (size_t)j * nr + r : sink;
+        *dst = f;
+      }
+      fetch_q(k + kDepth, qv[d], fl[d]);
+    }
   }
-  __syncthreads();
-  const double ir4 = a.vecs[V_IR4A4 * NRP + ii];
-  const double ir2 = a.vecs[V_IR2A4 * NRP + ii];
-  const double d2ii = sD2T[ii * NRS + ii];
-  const double pdt = a.Pr * a.dt;
-  double fe = 0.0, bf = 0.0;
-  load_row<NRS>(La, a.LinvA4 + (size_t)(K - 1 - start) * MS, NRP, ii);
-  int row = start;
-  for (int s = 0; s + 1 < nsteps; s += 2) {
-    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, La, Lb, sD2T, d2ii,
-                 ir4, ir2, gcur, fe, bf, fprev, sh);
-    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, Lb, La, sD2T, d2ii,
-                 ir4, ir2, gcur, fe, bf, fprev, sh);
+}
+
+template <int NRS, int NB>
+__device__ __forceinline__ void join_a4(const ChainArgs& a, int chain, int start, int seg,
+                                        int k0, int k1, const Inst<NB>& in,
+                                        double (*xs)[NB][2 * NRS + 64]) {
+  using S = CShape<NRS>;
+  constexpr int NQ2 = 2 * S::NQ;          // loads per lane of an NRP x 2NRS matrix
+  constexpr int CPL2 = 2 * S::CPL;        // entries of a 2NRS vector per lane
+  constexpr int BLK2 = 2 * S::BLK;        // doubles per packed NRP x 2NRS matrix
+  const int K = a.K, nr = a.nr;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r = 16 * w + (lane >> 2), kq = lane & 3;
+  const bool rv = r < nr;
+  const int rr = rv ? r : nr - 1;
+  const bool writer = kq == 0 && r < NRS;
+  const int wpf = writer ? vpos(r, CPL2) : 2 * NRS + lane;
+  const int wpb = writer ? vpos(NRS + r, CPL2) : 2 * NRS + lane;
+  double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
+  // state (f_e, bf_e): column c < NRS is f_e[c], column NRS + c is bf_e[c];
+  // E is stored as its f_e rows then its bf_e rows.
+  const double* eb = a.Eq[0] + (size_t)chain * a.nseg * 2 * BLK2 + w * NQ2 * 128 + 2 * lane;
+  double2 ef[kDepthE][NQ2], ebv[kDepthE][NQ2];
+  double exf[kDepthE][NB], exb[kDepthE][NB];
+  auto fetch_e = [&](int u, double2 (&qf)[NQ2], double2 (&qb)[NQ2], double (&xf)[NB],
+                     double (&xb)[NB]) {
+    const int uc = u < seg ? u : seg - 1;
+    load_blk<NQ2>(qf, eb + (size_t)uc * 2 * BLK2);
+    load_blk<NQ2>(qb, eb + (size_t)uc * 2 * BLK2 + BLK2);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      xf[t] = in.ex[t][(size_t)uc * 2 * S::NRP + rr];
+      xb[t] = in.ex[t][(size_t)uc * 2 * S::NRP + S::NRP + rr];
+    }
+  };
+#pragma unroll
+  for (int d = 0; d < kDepthE; ++d) fetch_e(d, ef[d], ebv[d], exf[d], exb[d]);
+  double sf[NB], sb[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) sf[t] = sb[t] = 0.0;
+  for (int ub = 0; ub < seg; ub += kDepthE) {
+#pragma unroll
+    for (int d = 0; d < kDepthE; ++d) {
+      const bool live = ub + d < seg;
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+      {
+        // lanes without an entry write their private tail slot (no branch)
+        xs[d & 1][t][wpf] = sf[t];
+        xs[d & 1][t][wpb] = sb[t];
+      }
+      lds_barrier();
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double* xv = &xs[d & 1][t][kq * CPL2];
+        const double nf = exf[d][t] + quad_sum(pdot<NQ2>(ef[d], xv));
+        const double nb = exb[d][t] + quad_sum(pdot<NQ2>(ebv[d], xv));
+        sf[t] = live ? nf : sf[t];
+        sb[t] = live ? nb : sb[t];
+      }
+      fetch_e(ub + d + kDepthE, ef[d], ebv[d], exf[d], exb[d]);
+    }
   }
-  if (nsteps & 1)
-    a4_step<NRS>(row, K, nr, i, active, pdt, g, x, Pp, a.LinvA4, MS, NRP, ii, La, Lb, sD2T, d2ii,
-                 ir4, ir2, gcur, fe, bf, fprev, sh);
+#pragma unroll
+  for (int t = 0; t < NB; ++t)
+  {
+    xs[0][t][wpf] = sf[t];
+    xs[0][t][wpb] = sb[t];
+  }
+  lds_barrier();
+  double xl[NB][CPL2];
+#pragma unroll
+  for (int t = 0; t < NB; ++t)
+#pragma unroll
+    for (int q = 0; q < CPL2; ++q) xl[t][q] = xs[0][t][kq * CPL2 + q];
+  // corrections f_row += Q_row s, and P_row = f_e of the row
+  const double* qb = a.Qq[0] + w * NQ2 * 128 + 2 * lane;
+  double2 qv[kDepth][NQ2];
+  double fl[kDepth][NB];
+  auto fetch_q = [&](int k, double2 (&qq)[NQ2], double (&f)[NB]) {
+    const int j = start - 2 * (k < k1 ? k : k1 - 1);
+    load_blk<NQ2>(qq, qb + (size_t)(K - 1 - j) * BLK2);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) f[t] = in.x[t][(size_t)j * nr + rr];
+  };
+#pragma unroll
+  for (int d = 0; d < kDepth; ++d) fetch_q(k0 + d, qv[d], fl[d]);
+  double fe[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) fe[t] = sf[t];
+  for (int kb = k0; kb < k1; kb += kDepth) {
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) {
+      const int k = kb + d;
+      const bool live = k < k1;
+      const int j = start - 2 * (live ? k : k1 - 1);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double f = fl[d][t] + quad_sum(pdot_reg<NQ2>(qv[d], xl[t]));
+        double* dst = (live && rv) ? in.x[t] + (size_t)j * nr + r : sink;
+        double* pdst = (live && rv) ? in.p[t] + (size_t)j * nr + r : sink;
+        *dst = f;
+        *pdst = fe[t];
+        fe[t] = live ? fe[t] + f : fe[t];
+      }
+      fetch_q(k + kDepth, qv[d], fl[d]);
+    }
+  }
+}
+
+template <int NRS, int NB>
+__global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_join(ChainArgs a) {
+  __shared__ __align__(16) double xs[2][NB][2 * NRS + 64];
+  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y;
+  if (seg == 0 || !((a.fields_mask >> field) & 1)) return;
+  if (a.symmetric && (field == 0 ? chain == 1 : chain == 0)) return;
+  const int start = a.K - 1 - chain;
+  const int n = start / 2 + 1;
+  const int k0 = seg * a.m;
+  if (k0 >= n) return;
+  const int k1 = k0 + a.m < n ? k0 + a.m : n;
+  const Inst<NB> in = instances<NB>(a, field, chain);
+  if (field == 0)
+    join_a4<NRS, NB>(a, chain, start, seg, k0, k1, in, xs);
+  else
+    join_nab2<NRS, NB>(a, field, chain, start, seg, k0, k1, in, xs);
+}
+
+template <int NRS, int NB>
+static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
+  using S = CShape<NRS>;
+  const dim3 grid(6, a.nseg, (B + NB - 1) / NB), block(64 * S::WR);
+  hipLaunchKernelGGL((k_chain_local<NRS, NB>), grid, block, 0, s, a);
+  if (int rc = hip_check(hipGetLastError(), "k_chain_local")) return rc;
+  if (a.nseg > 1) {
+    hipLaunchKernelGGL((k_chain_join<NRS, NB>), grid, block, 0, s, a);
+    if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
+  }
+  return SDDC_OK;
+}
+
+template <int NRS>
+static int dispatch_nb(const ChainArgs& a, int NB, int B, hipStream_t s) {
+  switch (NB) {
+    case 1: return launch_chain_t<NRS, 1>(a, B, s);
+    case 2: return launch_chain_t<NRS, 2>(a, B, s);
+    case 4: return launch_chain_t<NRS, 4>(a, B, s);
+    default: return fail(SDDC_ERR_INVALID, "unsupported instances per workgroup");
+  }
+}
+
+static int dispatch_chain(const ChainArgs& a, int nrs, int NB, int B, hipStream_t s) {
+  switch (nrs) {
+    case 8: return dispatch_nb<8>(a, NB, B, s);
+    case 16: return dispatch_nb<16>(a, NB, B, s);
+    case 24: return dispatch_nb<24>(a, NB, B, s);
+    case 32: return dispatch_nb<32>(a, NB, B, s);
+    case 40: return dispatch_nb<40>(a, NB, B, s);
+    case 48: return dispatch_nb<48>(a, NB, B, s);
+    case 56: return dispatch_nb<56>(a, NB, B, s);
+    case 64: return dispatch_nb<64>(a, NB, B, s);
+    default: return fail(SDDC_ERR_UNSUPPORTED, "unsupported radial size");
+  }
 }
 
 int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
                     const long long goff[3], double* X, long long xstride,
                     const long long xoff[3], double* Pout, int fields_mask, hipStream_t s) {
   if (B <= 0) return SDDC_OK;
-  SolveArgs a;
+  if (B > p.ex_batch) return fail(SDDC_ERR_INVALID, "batch exceeds the plan's solve workspace");
+  ChainArgs a;
   a.G = G;
   a.X = X;
-  a.Pout = Pout;
-  a.LinvA4 = p.LinvA4;
-  a.LinvT = p.LinvT;
-  a.LinvS = p.LinvS;
-  a.D2A4 = p.D2A4;
-  a.vecs = p.vecs;
-  a.gstride = gstride;
-  a.xstride = xstride;
+  a.P = Pout;
   for (int f = 0; f < 3; ++f) {
+    a.Lq[f] = p.chain_L[f];
+    a.Qq[f] = p.chain_Q[f];
+    a.Eq[f] = p.chain_E[f];
     a.goff[f] = goff[f];
     a.xoff[f] = xoff[f];
   }
+  a.D2q = p.chain_D2;
+  a.vecs = p.vecs;
+  a.EX = p.chain_EX;
+  a.scratch = p.solve_scratch;
+  a.gstride = gstride;
+  a.xstride = xstride;
   a.K = p.K;
   a.nr = p.nr;
-  a.NRP = p.NRP;
+  a.B = B;
+  a.m = p.seg_m;
+  a.nseg = p.nseg;
   a.symmetric = p.symmetric;
   a.fields_mask = fields_mask;
-  a.dt = p.dt;
-  a.Pr = p.Pr;
-  a.Tau = p.Tau;
-  const dim3 grid(6, B);
-  const int nrs = ((p.nr + 7) / 8) * 8;
-  switch (nrs) {
-    case 8: hipLaunchKernelGGL(k_solve<8>, grid, dim3(64), 0, s, a); break;
-    case 16: hipLaunchKernelGGL(k_solve<16>, grid, dim3(64), 0, s, a); break;
-    case 24: hipLaunchKernelGGL(k_solve<24>, grid, dim3(64), 0, s, a); break;
-    case 32: hipLaunchKernelGGL(k_solve<32>, grid, dim3(64), 0, s, a); break;
-    case 40: hipLaunchKernelGGL(k_solve<40>, grid, dim3(64), 0, s, a); break;
-    case 48: hipLaunchKernelGGL(k_solve<48>, grid, dim3(64), 0, s, a); break;
-    case 56: hipLaunchKernelGGL(k_solve<56>, grid, dim3(64), 0, s, a); break;
-    case 64: hipLaunchKernelGGL(k_solve<64>, grid, dim3(64), 0, s, a); break;
-    default: return fail(SDDC_ERR_UNSUPPORTED, "unsupported radial size");
-  }
-  return hip_check(hipGetLastError(), "k_solve");
+  a.dtf[0] = p.Pr * p.dt;
+  a.dtf[1] = p.dt;
+  a.dtf[2] = p.Tau * p.dt;   // NAB2 of S uses Tau*dt (Main.py:281)
+  int NB = B >= 64 ? 4 : (B >= 8 ? 2 : 1);
+  if (p.tune_solve_nbi > 0) NB = p.tune_solve_nbi;
+  return dispatch_chain(a, p.NRS, NB, B, s);
 }
 
 int launch_solve(const Plan& p, int B, const double* G, double* Xout, double* Pout,
@@ -256,6 +632,181 @@ int launch_solve(const Plan& p, int B, const double* G, double* Xout, double* Po
   const long long N = (long long)p.K * p.nr;
   const long long off[3] = {0, N, 2 * N};
   return launch_solve_ex(p, B, G, 3 * N, off, Xout, 3 * N, off, Pout, fields_mask, s);
+}
+
+// ---------------------------------------------------------------------------
+// Host side: packing and the segment maps (built once per plan, float64).
+// ---------------------------------------------------------------------------
+// Quad packing of an nrows x ncols row-major matrix (leading dimension lda)
+// into an NRP x NC tile (zero padded).
+static void pack_quad(const double* A, int nrows, int ncols, int lda, int NRP, int NC,
+                      double* out) {
+  const int NQ = NC / 8;
+  for (int w = 0; w < NRP / 16; ++w)
+    for (int q = 0; q < NQ; ++q)
+      for (int l = 0; l < 64; ++l)
+        for (int h = 0; h < 2; ++h) {
+          const int r = 16 * w + l / 4, c = (l & 3) + 8 * q + 4 * h;
+          out[((size_t)(w * NQ + q) * 64 + l) * 2 + h] =
+              (r < nrows && c < ncols) ? A[(size_t)r * lda + c] : 0.0;
+        }
+}
+
+// C (m x n) = A (m x k) B (k x n), row-major.
+static void matmul(const double* A, const double* B, double* C, int m, int k, int n) {
+  for (int i = 0; i < m; ++i) {
+    double* c = C + (size_t)i * n;
+    for (int j = 0; j < n; ++j) c[j] = 0.0;
+    for (int q = 0; q < k; ++q) {
+      const double av = A[(size_t)i * k + q];
+      const double* b = B + (size_t)q * n;
+      for (int j = 0; j < n; ++j) c[j] += av * b[j];
+    }
+  }
+}
+
+// An nr x 2nr state matrix [F | B] packed as NRP x 2NRS, with columns of the
+// second half starting at NRS (the kernels' state layout).
+static void pack_state(const double* A, int nr, int NRS, int NRP, double* out) {
+  std::vector<double> wide((size_t)nr * 2 * NRS, 0.0);
+  for (int i = 0; i < nr; ++i)
+    for (int c = 0; c < nr; ++c) {
+      wide[(size_t)i * 2 * NRS + c] = A[(size_t)i * 2 * nr + c];
+      wide[(size_t)i * 2 * NRS + NRS + c] = A[(size_t)i * 2 * nr + nr + c];
+    }
+  pack_quad(wide.data(), nr, 2 * NRS, 2 * NRS, NRP, 2 * NRS, out);
+}
+
+// Rows per segment for chains of n rows.  Small batches are latency bound
+// (depth m + n/m, smallest near m = sqrt(n)); mid-size batches trade depth for
+// fewer corrections (m = 2 sqrt(n)); large batches fill the device without
+// segments, which avoid the corrections' extra work (measured at n = 256 on
+// MI355X).  SDDC_CHAIN_SEGMENT overrides.  Fixed per plan, so that the bits
+// of a result do not depend on the batch it was computed in.
+static int default_segment(int n, int max_batch) {
+  if (const char* env = std::getenv("SDDC_CHAIN_SEGMENT")) {
+    const int m = std::atoi(env);
+    if (m > 0) return m;
+  }
+  if (max_batch > 96) return n;
+  const double c = max_batch <= 4 ? 1.0 : 2.0;
+  int m = (int)std::ceil(c * std::sqrt((double)n));
+  m = ((m + kDepth - 1) / kDepth) * kDepth;
+  return m < 8 ? 8 : m;
+}
+
+template <typename T>
+static int upload_vec(T** dst, const std::vector<T>& v) {
+  SDDC_HIP(hipMalloc(dst, v.size() * sizeof(T)));
+  SDDC_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return SDDC_OK;
+}
+
+int build_chain(Plan& p, const sddc_operators* ops, int max_batch) {
+  const int K = p.K, nr = p.nr, NRS = p.NRS, NRP = p.NRP;
+  const size_t BLK = (size_t)NRP * NRS, NN = (size_t)nr * nr;
+  const int n = K / 2;   // rows per chain (K is even)
+  int m = default_segment(n, max_batch);
+  if (m > n) m = n;
+  p.seg_m = m;
+  p.nseg = (n + m - 1) / m;
+  const double* inv[3] = {ops->Linv_A4, ops->Linv_T, ops->Linv_S};
+  std::vector<double> buf((size_t)K * BLK);
+  for (int f = 0; f < 3; ++f) {
+    for (int jj = 0; jj < K; ++jj)
+      pack_quad(inv[f] + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
+    if (int rc = upload_vec(&p.chain_L[f], buf)) return rc;
+  }
+  std::vector<double> d2(BLK);
+  pack_quad(ops->D2_A4, nr, nr, nr, NRP, NRS, d2.data());
+  if (int rc = upload_vec(&p.chain_D2, d2)) return rc;
+  if (p.nseg == 1) return SDDC_OK;   // one segment per chain: no joins
+
+  // NAB2 (fields 1, 2).  State b; T maps the entry state to b at the row:
+  // Q = -c_j Linv_j T, T <- T + 2 dtf j Q; E = T after the segment.
+  const double dts[3] = {p.Pr * p.dt, p.dt, p.Tau * p.dt};
+  std::vector<double> T(NN), Q(NN), LT(NN);
+  for (int f = 1; f < 3; ++f) {
+    std::vector<double> qpk((size_t)K * BLK, 0.0), epk((size_t)2 * p.nseg * BLK, 0.0);
+    for (int chain = 0; chain < 2; ++chain) {
+      const int start = K - 1 - chain;
+      for (int s = 0; s < p.nseg; ++s) {
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int i = 0; i < nr; ++i) T[(size_t)i * nr + i] = 1.0;
+        const int k0 = s * m, k1 = std::min(k0 + m, n);
+        for (int k = k0; k < k1; ++k) {
+          const int j = start - 2 * k, jj = K - 1 - j;
+          const double cj = (j == 0) ? 0.5 : 1.0;
+          matmul(inv[f] + jj * NN, T.data(), LT.data(), nr, nr, nr);
+          for (size_t e = 0; e < NN; ++e) Q[e] = -cj * LT[e];
+          pack_quad(Q.data(), nr, nr, nr, NRP, NRS, qpk.data() + jj * BLK);
+          const double alpha = 2.0 * dts[f] * j;
+          for (size_t e = 0; e < NN; ++e) T[e] += alpha * Q[e];
+        }
+        pack_quad(T.data(), nr, nr, nr, NRP, NRS, epk.data() + ((size_t)chain * p.nseg + s) * BLK);
+      }
+    }
+    if (int rc = upload_vec(&p.chain_Q[f], qpk)) return rc;
+    if (int rc = upload_vec(&p.chain_E[f], epk)) return rc;
+  }
+
+  // A4 (field 0).  State (f_e, bf_e), 2nr; Tf, Tb (nr x 2nr) map the entry
+  // state to f_e and bf_e at the row.  With j = row + 1, bj = -j(j+1),
+  // bjt = -2j:  M T = dt bjt (D2 Tf + bj IR4 Tf + IR4 Tb) - bjt IR2 Tf,
+  // Q = Linv M T,  Tb <- Tb + bj Q + bjt Tf,  Tf <- Tf + Q.
+  {
+    const size_t NW = (size_t)nr * 2 * nr, BLK2 = 2 * BLK;
+    const double dta = dts[0];
+    std::vector<double> Tf(NW), Tb(NW), D2T(NW), MT(NW), Q2(NW);
+    std::vector<double> qpk((size_t)K * BLK2, 0.0), epk((size_t)2 * p.nseg * 2 * BLK2, 0.0);
+    for (int chain = 0; chain < 2; ++chain) {
+      const int start = K - 1 - chain;
+      for (int s = 0; s < p.nseg; ++s) {
+        std::fill(Tf.begin(), Tf.end(), 0.0);
+        std::fill(Tb.begin(), Tb.end(), 0.0);
+        for (int i = 0; i < nr; ++i) {
+          Tf[(size_t)i * 2 * nr + i] = 1.0;
+          Tb[(size_t)i * 2 * nr + nr + i] = 1.0;
+        }
+        const int k0 = s * m, k1 = std::min(k0 + m, n);
+        for (int k = k0; k < k1; ++k) {
+          const int row = start - 2 * k, jj = K - 1 - row;
+          const double jp = row + 1.0, bj = -jp * (jp + 1.0), bjt = -2.0 * jp;
+          matmul(ops->D2_A4, Tf.data(), D2T.data(), nr, nr, 2 * nr);
+          for (int i = 0; i < nr; ++i) {
+            const double ir4 = ops->IR4_A4[i], ir2 = ops->IR2_A4[i];
+            for (int c = 0; c < 2 * nr; ++c) {
+              const size_t e = (size_t)i * 2 * nr + c;
+              MT[e] = dta * bjt * (D2T[e] + bj * ir4 * Tf[e] + ir4 * Tb[e]) - bjt * ir2 * Tf[e];
+            }
+          }
+          matmul(inv[0] + jj * NN, MT.data(), Q2.data(), nr, nr, 2 * nr);
+          pack_state(Q2.data(), nr, NRS, NRP, qpk.data() + jj * BLK2);
+          for (size_t e = 0; e < NW; ++e) {
+            Tb[e] += bj * Q2[e] + bjt * Tf[e];
+            Tf[e] += Q2[e];
+          }
+        }
+        double* eo = epk.data() + ((size_t)chain * p.nseg + s) * 2 * BLK2;
+        pack_state(Tf.data(), nr, NRS, NRP, eo);
+        pack_state(Tb.data(), nr, NRS, NRP, eo + BLK2);
+      }
+    }
+    if (int rc = upload_vec(&p.chain_Q[0], qpk)) return rc;
+    if (int rc = upload_vec(&p.chain_E[0], epk)) return rc;
+  }
+  return SDDC_OK;
+}
+
+void free_chain(Plan& p) {
+  for (int f = 0; f < 3; ++f) {
+    if (p.chain_L[f]) (void)hipFree(p.chain_L[f]);
+    if (p.chain_Q[f]) (void)hipFree(p.chain_Q[f]);
+    if (p.chain_E[f]) (void)hipFree(p.chain_E[f]);
+    p.chain_L[f] = p.chain_Q[f] = p.chain_E[f] = nullptr;
+  }
+  if (p.chain_D2) (void)hipFree(p.chain_D2);
+  p.chain_D2 = nullptr;
 }
 
 }  // namespace sddc

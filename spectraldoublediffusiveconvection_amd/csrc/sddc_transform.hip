@@ -14,7 +14,7 @@ __global__ __launch_bounds__(256) void k_transform(FftShape sh, const double2* t
                                                    const double* __restrict__ in,
                                                    double* __restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ double red[8];
+  __shared__ int red[16 * 2];
   using C = cplx<double>;
   C* bufA = reinterpret_cast<C*>(smem_raw);
   C* bufB = bufA + sh.L;
@@ -30,9 +30,9 @@ __global__ __launch_bounds__(256) void k_transform(FftShape sh, const double2* t
     const bool is_sin = (kind == SDDC_IDST);
     // IDCT with n < len_in keeps the first n coefficients (DCT-III truncation)
     const int K = len_in < L ? len_in : L;
-    double mx[2] = {series_absmax<double>(u, K, is_sin, 0), series_absmax<double>(v, K, is_sin, 0)};
-    block_max<2, double>(mx, red);
-    const double su = pow2_scale(mx[0]), sv = pow2_scale(mx[1]);
+    int mx[2] = {series_maxexp<double>(u, K, is_sin, 0), series_maxexp<double>(v, K, is_sin, 0)};
+    block_max_exp<2>(mx, red);
+    const double su = pow2_scale_e<double>(mx[0]), sv = pow2_scale_e<double>(mx[1]);
     syn_pack<double>(bufA, u, is_sin, 0, v, is_sin, 0, K, L, rot, su, sv);
     __syncthreads();
     C* res = fft_run<double>(bufA, bufB, 1, sh, tw, true);
@@ -45,13 +45,13 @@ __global__ __launch_bounds__(256) void k_transform(FftShape sh, const double2* t
     }
   } else {
     const bool is_sin = (kind == SDDC_DST);
-    double mx[2] = {0.0, 0.0};
+    int mx[2] = {-1000, -1000};
     for (int q = threadIdx.x; q < L; q += blockDim.x) {
-      mx[0] = fmax(mx[0], fabs(u[q]));
-      if (v) mx[1] = fmax(mx[1], fabs(v[q]));
+      mx[0] = max(mx[0], bexp(u[q]));
+      if (v) mx[1] = max(mx[1], bexp(v[q]));
     }
-    block_max<2, double>(mx, red);
-    const double su = pow2_scale(mx[0]), sv = pow2_scale(mx[1]);
+    block_max_exp<2>(mx, red);
+    const double su = pow2_scale_e<double>(mx[0]), sv = pow2_scale_e<double>(mx[1]);
     for (int q = threadIdx.x; q < L; q += blockDim.x) {
       const int n = grid_index(q, L);
       const double sg = is_sin ? (double)grid_sign(q, L) : 1.0;
@@ -142,12 +142,12 @@ __device__ __forceinline__ void field_src(const SlabRows<R>& sr, int f, const R*
 }
 
 template <typename R>
-__device__ __forceinline__ R field_absmax(const SlabRows<R>& sr, int f, int K) {
+__device__ __forceinline__ int field_maxexp(const SlabRows<R>& sr, int f, int K) {
   const R* row;
   bool is_sin;
   int mult;
   field_src(sr, f, row, is_sin, mult);
-  return series_absmax<R>(row, K, is_sin, mult);
+  return series_maxexp<R>(row, K, is_sin, mult);
 }
 
 template <typename R>
@@ -191,19 +191,19 @@ __device__ void project_and_store(cplx<R>* Z, int L, int K, const cplx<R>* rot, 
 // Scale the two product pairs (Q,R) at O and (NT,NS) at O+L to unit
 // magnitude before the forward FFT; returns the unscaling factors.
 template <typename R>
-__device__ __forceinline__ void scale_products(cplx<R>* O, int L, R* red, R (&inv)[4]) {
-  R mx[4] = {(R)0, (R)0, (R)0, (R)0};
+__device__ __forceinline__ void scale_products(cplx<R>* O, int L, int* red, R (&inv)[4]) {
+  int mx[4] = {-1000, -1000, -1000, -1000};
   for (int q = threadIdx.x; q < L; q += blockDim.x) {
-    mx[0] = fmax(mx[0], fabs(O[q].x));
-    mx[1] = fmax(mx[1], fabs(O[q].y));
-    mx[2] = fmax(mx[2], fabs(O[L + q].x));
-    mx[3] = fmax(mx[3], fabs(O[L + q].y));
+    mx[0] = max(mx[0], bexp(O[q].x));
+    mx[1] = max(mx[1], bexp(O[q].y));
+    mx[2] = max(mx[2], bexp(O[L + q].x));
+    mx[3] = max(mx[3], bexp(O[L + q].y));
   }
-  block_max<4, R>(mx, red);
+  block_max_exp<4>(mx, red);
   R sc[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    sc[k] = pow2_scale(mx[k]);
+    sc[k] = pow2_scale_e<R>(mx[k]);
     inv[k] = (R)1 / sc[k];
   }
   for (int q = threadIdx.x; q < L; q += blockDim.x) {
@@ -214,13 +214,13 @@ __device__ __forceinline__ void scale_products(cplx<R>* O, int L, R* red, R (&in
 }
 
 template <typename R, bool DFX>
-__global__ __launch_bounds__(256) void k_nonlinear(FftShape sh, const cplx<R>* __restrict__ tw_g,
+__global__ __launch_bounds__(1024) void k_nonlinear(FftShape sh, const cplx<R>* __restrict__ tw_g,
                                                    const cplx<R>* __restrict__ rot_g, int K, int nr,
                                                    const R* __restrict__ slabs,
                                                    const R* __restrict__ slabs2,
                                                    R* __restrict__ F) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ R red[4 * 18];
+  __shared__ int red[16 * 18];
   using C = cplx<R>;
   const int L = sh.L;
   C* A = reinterpret_cast<C*>(smem_raw);
@@ -240,11 +240,9 @@ __global__ __launch_bounds__(256) void k_nonlinear(FftShape sh, const cplx<R>* _
   // chains in the synthesis loops.
   const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
   R* stage = reinterpret_cast<R*>(Bf);
-#pragma unroll 8
-  for (int e = threadIdx.x; e < 7 * K; e += blockDim.x) {
-    const int r = e / K;
-    stage[e] = sg.s[r][e - r * K];
-  }
+#pragma unroll
+  for (int r = 0; r < 7; ++r)
+    for (int k = threadIdx.x; k < K; k += blockDim.x) stage[r * K + k] = sg.s[r][k];
   SlabRows<R> sr;
 #pragma unroll
   for (int r = 0; r < 7; ++r) sr.s[r] = stage + r * K;
@@ -253,17 +251,17 @@ __global__ __launch_bounds__(256) void k_nonlinear(FftShape sh, const cplx<R>* _
   __syncthreads();
 
   // power-of-two scales of the nine fields of X (and of dv)
-  R mx[18];
+  int mx[18];
 #pragma unroll
   for (int f = 0; f < 9; ++f) {
-    mx[f] = field_absmax<R>(sr, f, K);
-    mx[9 + f] = DFX ? field_absmax<R>(sd, f, K) : (R)0;
+    mx[f] = field_maxexp<R>(sr, f, K);
+    mx[9 + f] = DFX ? field_maxexp<R>(sd, f, K) : -1000;
   }
-  block_max<18, R>(mx, red);
+  block_max_exp<18>(mx, red);
   R sc[18], is[18];
 #pragma unroll
   for (int f = 0; f < 18; ++f) {
-    sc[f] = pow2_scale(mx[f]);
+    sc[f] = pow2_scale_e<R>(mx[f]);
     is[f] = (R)1 / sc[f];
   }
 
@@ -368,11 +366,11 @@ template <typename R, bool DFX>
 static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
                        hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
   const size_t lds = 12 * (size_t)p.M * sizeof(cplx<R>);
-  if (lds + 4 * 18 * sizeof(R) > 160 * 1024)
+  if (lds + 16 * 18 * sizeof(int) > 160 * 1024)
     return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
   static size_t cur = 0;
   if (int rc = ensure_dynamic_lds((const void*)k_nonlinear<R, DFX>, lds, cur)) return rc;
-  hipLaunchKernelGGL((k_nonlinear<R, DFX>), dim3(p.nr, B), dim3(256), lds, s, p.tabM.shape, tw,
+  hipLaunchKernelGGL((k_nonlinear<R, DFX>), dim3(p.nr, B), dim3(1024), lds, s, p.tabM.shape, tw,
                      rot, p.K, p.nr, (const R*)slabs, (const R*)slabs2, (R*)F);
   return hip_check(hipGetLastError(), "k_nonlinear");
 }
@@ -407,7 +405,7 @@ __global__ __launch_bounds__(256) void k_ke_rows(FftShape sh, const double2* tw_
   extern __shared__ __align__(16) unsigned char smem_raw[];
   using C = cplx<double>;
   __shared__ double red[256];
-  __shared__ double red2[8];
+  __shared__ int red2[16 * 2];
   const int L = sh.L;
   C* A = reinterpret_cast<C*>(smem_raw);
   C* Bf = A + L;
@@ -418,13 +416,13 @@ __global__ __launch_bounds__(256) void k_ke_rows(FftShape sh, const double2* tw_
   const double* dp = slabs + ((size_t)(b * 7 + 1) * nr + i) * K;
   // IR@JPSI = (1/R_i) * JPSI (Main.py:102,112)
   const double ir = vecs[V_IR * NRP + i];
-  double mx[2] = {0.0, 0.0};
+  int mx[2] = {-1000, -1000};
   for (int k = threadIdx.x; k < K; k += blockDim.x) {
-    mx[0] = fmax(mx[0], fabs(ir * jt[k]));
-    if (k > 0) mx[1] = fmax(mx[1], fabs(dp[k]));
+    mx[0] = max(mx[0], bexp(ir * jt[k]));
+    if (k > 0) mx[1] = max(mx[1], bexp(dp[k]));
   }
-  block_max<2, double>(mx, red2);
-  const double su = pow2_scale(mx[0]), sv = pow2_scale(mx[1]);
+  block_max_exp<2>(mx, red2);
+  const double su = pow2_scale_e<double>(mx[0]), sv = pow2_scale_e<double>(mx[1]);
   for (int k = threadIdx.x; k < L; k += blockDim.x) {
     const double au = k < K ? (k == 0 ? ir * jt[0] : 0.5 * (ir * jt[k])) : 0.0;
     const double aum = (k > 0 && L - k < K) ? 0.5 * (ir * jt[L - k]) : 0.0;
