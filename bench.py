@@ -49,6 +49,21 @@ def seeded_states(B, n3, seed=1234):
     return 1e-3 * X / np.linalg.norm(X, axis=1, keepdims=True)
 
 
+def measured_traffic(kernel, B):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs (scaled by a known-size copy), when they were
+    taken on this workload; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not rec or rec.get("workload") != "N_r=40 N_fm=512 B=%d mixed" % B:
+        return None
+    return rec["traffic_bytes"]
+
+
 def cpu_baseline(seconds):
     """The oracle's float64 NumPy step (reference algorithm restated) on one
     host core, over a bounded sample of the same workload."""
@@ -110,6 +125,7 @@ def main():
 
     # ---- per-kernel timing of the dominant kernel (HIP events on its stream)
     roof = profiling.roofline(plan, X, Ra, w["Ra_s"], reps=20)
+    roof["traffic"] = measured_traffic(roof["kernel"], B)
 
     # ---- plain float64 single-instance step (eager launches)
     p64 = dev.Plan(op)

@@ -118,9 +118,10 @@ int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
 int sddc_step_linearized(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
                          const double* dv, const double* X, double* out, void* stream);
 
-/* Diagnostics of the time loop (Main.py:292-295): out [B][4] =
- * (||X||_2, Kinetic_Energy (Main.py:71-134), Nusselt(T) and Nusselt(S)
- * (Main.py:41-68, value at R_1, no check)). */
+/* Diagnostics of the time loop (Main.py:292-295): out [B][6] =
+ * (||X||_2, Kinetic_Energy (Main.py:71-134), Nusselt(T) and Nusselt(S) at R_1
+ * (Main.py:41-68, the value Nusselt returns), Nusselt(T) and Nusselt(S) at R_2
+ * (the other end of NuM1, which Nusselt's check compares, Main.py:63-66)). */
 int sddc_diagnostics(sddc_plan* plan, int B, const double* X, double* out, void* stream);
 
 /* Capture nsteps of sddc_step for a fixed batch (pointers fixed) into a HIP

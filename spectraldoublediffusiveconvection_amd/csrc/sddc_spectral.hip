@@ -473,32 +473,40 @@ int launch_mask(const Plan& p, int B, const double* X, double* out, hipStream_t 
 
 // ---------------------------------------------------------------------------
 // Diagnostics: ||X||_2, kinetic energy (radial trapezoid over the interior
-// points of the per-row theta integrals, Main.py:129-134) and Nu-1 at R_1 for
-// T and S (Main.py:52-68).  One workgroup per instance.
+// points of the per-row theta integrals, Main.py:129-134) and Nu-1 of T and S
+// at R_1 and at R_2 (Main.py:52-68; the reference returns the R_1 value and
+// checks it against R_2).  One workgroup per instance; out[b] holds
+// (norm, KE, NuT(R_1), NuS(R_1), NuT(R_2), NuS(R_2)).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ X,
                                                     const double* __restrict__ rowval,
                                                     const double* __restrict__ Dfull,
                                                     const double* __restrict__ Rg, int K, int nr,
                                                     double d, double* __restrict__ out) {
-  __shared__ double red[4][256];
+  constexpr int NV = 6;
+  __shared__ double red[NV][256];
   const int b = blockIdx.x;
   const size_t N = (size_t)K * nr;
   const double* Xb = X + (size_t)b * 3 * N;
   const int Np1 = nr + 2;
-  double s2 = 0.0, nuT = 0.0, nuS = 0.0, ke = 0.0;
-  for (size_t e = threadIdx.x; e < 3 * N; e += blockDim.x) s2 += Xb[e] * Xb[e];
-  // Nusselt: sum over even k of (D[0,1:-1] . T_k) / (1 - k^2)
+  const double* Dlast = Dfull + (size_t)(Np1 - 1) * Np1;   // row of R_2
+  double v[NV] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (size_t e = threadIdx.x; e < 3 * N; e += blockDim.x) v[0] += Xb[e] * Xb[e];
+  // Nusselt: sum over even k of (D[row,1:-1] . T_k) / (1 - k^2), rows 0 and N_r
   for (int k = 2 * threadIdx.x; k < K; k += 2 * blockDim.x) {
-    double aT = 0.0, aS = 0.0;
+    double aT = 0.0, aS = 0.0, bT = 0.0, bS = 0.0;
     for (int i = 0; i < nr; ++i) {
-      const double dd = Dfull[i + 1];
-      aT += dd * Xb[N + (size_t)k * nr + i];
-      aS += dd * Xb[2 * N + (size_t)k * nr + i];
+      const double t = Xb[N + (size_t)k * nr + i], c = Xb[2 * N + (size_t)k * nr + i];
+      aT += Dfull[i + 1] * t;
+      aS += Dfull[i + 1] * c;
+      bT += Dlast[i + 1] * t;
+      bS += Dlast[i + 1] * c;
     }
     const double den = 1.0 - (double)k * (double)k;
-    nuT += aT / den;
-    nuS += aS / den;
+    v[2] += aT / den;
+    v[3] += aS / den;
+    v[4] += bT / den;
+    v[5] += bS / den;
   }
   for (int i = threadIdx.x; i < nr; i += blockDim.x) {
     const double* x = Rg + 1;   // interior points
@@ -507,27 +515,27 @@ __global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ X
     else if (i == 0) w = 0.5 * (x[1] - x[0]);
     else if (i == nr - 1) w = 0.5 * (x[nr - 1] - x[nr - 2]);
     else w = 0.5 * (x[i + 1] - x[i - 1]);
-    if (rowval) ke += w * rowval[(size_t)b * nr + i];
+    if (rowval) v[1] += w * rowval[(size_t)b * nr + i];
   }
-  red[0][threadIdx.x] = s2;
-  red[1][threadIdx.x] = ke;
-  red[2][threadIdx.x] = nuT;
-  red[3][threadIdx.x] = nuS;
+  for (int q = 0; q < NV; ++q) red[q][threadIdx.x] = v[q];
   __syncthreads();
   for (int st = 128; st > 0; st >>= 1) {
     if (threadIdx.x < st)
-      for (int q = 0; q < 4; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+      for (int q = 0; q < NV; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + st];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     const double R1 = 1 / d, R2 = (1 + d) / d;
     const double A_T = (R1 * R2) / (R1 - R2);
     const double V = (2.0 / 3.0) * (Rg[Np1 - 1] * Rg[Np1 - 1] * Rg[Np1 - 1] - Rg[0] * Rg[0] * Rg[0]);
-    const double c0 = (Rg[0] * Rg[0]) / A_T;
-    out[(size_t)b * 4 + 0] = sqrt(red[0][0]);
-    out[(size_t)b * 4 + 1] = (0.5 / V) * red[1][0];
-    out[(size_t)b * 4 + 2] = c0 * red[2][0];
-    out[(size_t)b * 4 + 3] = c0 * red[3][0];
+    const double c0 = (Rg[0] * Rg[0]) / A_T, c1 = (Rg[Np1 - 1] * Rg[Np1 - 1]) / A_T;
+    double* o = out + (size_t)b * NV;
+    o[0] = sqrt(red[0][0]);
+    o[1] = (0.5 / V) * red[1][0];
+    o[2] = c0 * red[2][0];
+    o[3] = c0 * red[3][0];
+    o[4] = c1 * red[4][0];
+    o[5] = c1 * red[5][0];
   }
 }
 

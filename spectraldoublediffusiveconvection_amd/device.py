@@ -183,9 +183,10 @@ class Plan:
         return out
 
     def diagnostics(self, X):
-        """(B, 4): ||X||_2, kinetic energy, Nu_T - 1, Nu_S - 1 (Main.py:292-295)."""
+        """(B, 6): ||X||_2, kinetic energy, Nu_T - 1 and Nu_S - 1 at R_1 (the
+        values of Main.py:292-295), Nu_T - 1 and Nu_S - 1 at R_2."""
         X = self._state(X)
-        out = torch.empty((X.shape[0], 4), dtype=torch.float64, device=X.device)
+        out = torch.empty((X.shape[0], 6), dtype=torch.float64, device=X.device)
         _lib.check(_lib.lib().sddc_diagnostics(self._h, X.shape[0], _dev_ptr(X), _dev_ptr(out), _stream()))
         return out
 

@@ -1,0 +1,198 @@
+"""Drop-in for the reference's Main.py: operator assembly, diagnostics, the
+time stepper and the Newton solver, with every step, linearised step and
+diagnostic computed by libsddc on the GPU.
+
+Signatures and return values follow Main.py (file:line on each function).
+_Newton keeps the reference's Krylov method (scipy's lgmres on the host, the
+same arguments) around the device's step and linearised step; its iterates
+match the reference's up to rounding, and converged states agree to the
+Newton tolerance.  Checkpoints use h5py with the reference's groups when
+h5py is importable, otherwise a NumPy .npz beside the requested name.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from .. import operators as _ops
+from . import Matrix_Operators as _mo
+from . import _registry as _reg
+
+
+def _dev(a):
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(a, dtype=np.float64)), device="cuda").view(1, -1)
+
+
+def _host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()[0]
+
+
+def Base_State_Coeffs(d):
+    """T_0 = -A_T/r + B_T (Main.py:23-38)."""
+    return _ops.base_state_coeffs(d)
+
+
+def _nusselt_ends(T_hat, d, R, D, N_fm, nr):
+    N = N_fm * nr
+    x = np.zeros(3 * N)
+    x[N:2 * N] = np.asarray(T_hat, dtype=np.float64)[:N]
+    p = _reg.geometry_plan(D, R, N_fm, False, d=d)
+    out = _host(p.diagnostics(_dev(x)))
+    return out[2], out[4]
+
+
+def Nusselt(T_hat, d, R, D, N_fm, nr, check=True):
+    """Nu - 1 at R_1; with check, the relative difference to the value at R_2
+    must stay below 1e-3 (Main.py:42-68)."""
+    nu1, nu2 = _nusselt_ends(T_hat, d, R, D, N_fm, nr)
+    error = abs(nu2 - nu1) / abs(nu2)
+    if (check is True) and (error > 1e-03):
+        raise ValueError("|Nu(R_1) - Nu(R_2)|/|Nu(R_1)| = ", error, " > .1%, Increase Nr \n")
+    return nu1
+
+
+def Kinetic_Energy(X_hat, R, D, N_fm, nr, symmetric):
+    """Volume-averaged kinetic energy (Main.py:71-134)."""
+    p = _reg.geometry_plan(D, R, N_fm, symmetric)
+    return _host(p.diagnostics(_dev(X_hat)))[1]
+
+
+def Eq_SYM(X1, R):
+    """0/1 mask keeping odd psi rows and even T/S rows (Main.py:137-176)."""
+    X_sym = np.ones(np.shape(X1))
+    nr = len(R[1:-1])
+    N_fm = int(np.shape(X1)[0] / (3 * nr))
+    N = N_fm * nr
+    for k in range(N_fm):
+        if k % 2 == 0:
+            X_sym[k * nr:k * nr + nr] = 0.0
+        else:
+            X_sym[N + k * nr:N + k * nr + nr] = 0.0
+            X_sym[2 * N + k * nr:2 * N + k * nr + nr] = 0.0
+    return X_sym
+
+
+def Build_Matrix_Operators(N_fm, N_r, d, dt, Pr, Tau):
+    """(D, R, Rsq, DT0, gr_K, L_inv_A4, L_inv_NAB2_T, L_inv_NAB2_S, args_A4,
+    args_FXS) as Main.py:179-225 returns them (host float64)."""
+    ops = _ops.build(N_fm, N_r, d, dt, Pr, Tau)
+    _reg.remember_geometry(ops.D, ops.R)
+    nr = N_r - 1
+    Rsq = _mo.R2(ops.R, N_fm)
+    gr_K = _mo.kGR_RT(ops.R, N_fm, d)
+    IR2 = np.ascontiguousarray(np.diag(ops.IR2_A4))
+    IR4 = np.ascontiguousarray(np.diag(ops.IR4_A4))
+    args_A4 = (np.ascontiguousarray(ops.D2_A4), IR4, IR2, N_fm, nr)
+    args_FXS = (ops.D, ops.R, N_fm, nr)
+    return (ops.D, ops.R, Rsq, np.array(ops.DT0), gr_K, list(ops.Linv_A4), list(ops.Linv_T),
+            list(ops.Linv_S), args_A4, args_FXS)
+
+
+def _write_checkpoint(save_filename, X_DATA, scalars, params):
+    try:
+        import h5py
+    except ImportError:
+        h5py = None
+    if h5py is not None:
+        with h5py.File(save_filename, "w") as f:
+            f.create_group("Checkpoints")["X_DATA"] = X_DATA
+            g = f.create_group("Scalar_Data")
+            for k, v in scalars.items():
+                g[k] = v
+            g = f.create_group("Parameters")
+            for k, v in params.items():
+                g[k] = v
+        return save_filename
+    data = {"Checkpoints/X_DATA": np.asarray(X_DATA)}
+    data.update({"Scalar_Data/" + k: np.asarray(v) for k, v in scalars.items()})
+    data.update({"Parameters/" + k: np.asarray(v) for k, v in params.items()})
+    np.savez(save_filename + ".npz", **data)
+    return save_filename + ".npz"
+
+
+def _Time_Step(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, save_filename='TimeStep_0.h5',
+               start_time=0, Total_time=1, dt=1e-04, linear=False, Verbose=False):
+    """Integrate Total_time/dt steps of Step_Python with the loop's symmetry
+    mask, diagnostics every step and a checkpoint every tenth of the run
+    (Main.py:228-329).  Returns the last state."""
+    plan = _reg.step_plan(N_fm, N_r, d, dt, Pr, Tau, symmetric)
+    N_iters = int(Total_time / dt)
+    N_save = max(int(N_iters / 10), 1)
+    N_print = 10
+    x = _dev(X)
+    xn = torch.empty_like(x)
+    Time, X_DATA, KE, NuT, NuS, Norm = [], [], [], [], [], []
+    params = {"Ra": Ra, "Ra_s": Ra_s, "Tau": Tau, "Pr": Pr, "d": d, "N_r": N_r, "N_fm": N_fm,
+              "dt": dt, "start_time": start_time, "symmetric": symmetric}
+    timer = 0.0
+    X_new = np.asarray(X, dtype=np.float64)
+    for iteration in range(N_iters):
+        torch.cuda.synchronize()
+        st = time.time()
+        plan.step(x, Ra, Ra_s, nsteps=1, linear=linear, out=xn)
+        torch.cuda.synchronize()
+        end = time.time()
+        dg = _host(plan.diagnostics(xn))
+        Norm.append(dg[0])
+        KE.append(dg[1])
+        NuT.append(dg[2])
+        NuS.append(dg[3])
+        Time.append(start_time + dt * iteration)
+        if (iteration % N_print == 0) and (Verbose is True):
+            print("Iteration =%i, Time = %e, Energy = %e, NuT = %e \n " %
+                  (iteration, start_time + dt * iteration, KE[-1], NuT[-1]))
+        if iteration % N_save == 0:
+            X_DATA.append(_host(xn))
+            _write_checkpoint(save_filename, X_DATA, {"Norm": Norm, "KE": KE, "Nu_T": NuT,
+                                                      "Nu_S": NuS, "Time": Time}, params)
+        x, xn = xn, x   # the step already applied the symmetry mask
+        if iteration > 0:
+            timer += end - st
+    if N_iters > 1:
+        print("Avg time = %e \n" % (timer / (N_iters - 1)))
+    if N_iters > 0:
+        X_new = _host(x)
+    return X_new
+
+
+def _Newton(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt=1, tol_newton=1e-08,
+            tol_gmres=1e-04, Krylov_Space_Size=300):
+    """Steady state by Newton iteration on Step(X) - X with lgmres on the
+    linearised step (Main.py:430-554).  Returns (X, Norm, KE, NuT, NuS, True)
+    or (None, None, None, None, None, False)."""
+    import scipy.sparse.linalg as spla
+    plan = _reg.step_plan(N_fm, N_r, d, dt, Pr, Tau, symmetric)
+    ops = plan.ops
+    R, D = ops.R, ops.D
+    nr = N_r - 1
+    N = N_fm * nr
+    X = np.asarray(X, dtype=np.float64)
+    X_SYM = Eq_SYM(X, R) if symmetric is True else 1
+    error, iteration, exit = 1.0, 0, 1.0
+    while (error > tol_newton) and (iteration < 5):
+        X = X_SYM * X
+        xd = _dev(X)
+        fx = _host(plan.step(xd, Ra, Ra_s)) - X
+
+        def DF(dv, xd=xd):
+            dv = np.asarray(dv, dtype=np.float64).reshape(-1)
+            return _host(plan.step_linearized(_dev(dv), xd, Ra, Ra_s)) - dv
+
+        dfx = spla.LinearOperator((X.shape[0], X.shape[0]), matvec=DF, dtype='float64')
+        b_norm = np.linalg.norm(fx, 2)
+        dv, exit = spla.lgmres(dfx, fx, maxiter=250, inner_m=Krylov_Space_Size, atol=tol_gmres * b_norm)
+        X = X - dv
+        error = np.linalg.norm(dv, 2) / np.linalg.norm(X, 2)
+        print('Newton Iteration = %d, Error = %e' % (iteration, error), "\n")
+        iteration += 1
+    if (iteration == 5) or (exit != 0):
+        print('Newton iteration could not converge')
+        return None, None, None, None, None, False
+    Norm = np.linalg.norm(X, 2)
+    KE = Kinetic_Energy(X, R, D, N_fm, nr, symmetric)
+    NuT = Nusselt(X[N:2 * N], d, R, D, N_fm, nr)
+    NuS = Nusselt(X[2 * N:3 * N], d, R, D, N_fm, nr)
+    return X, Norm, KE, NuT, NuS, True

@@ -62,3 +62,18 @@ def roofline(plan, X, Ra, Ra_s, reps=20):
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": name,
             "avg_us": ms * 1e3, "bytes_per_launch": nbytes,
             "stages_us": {k: v[0] * 1e3 for k, v in tab.items()}}
+
+
+def copy_probe(nbytes=256 << 20):
+    """One device-to-device copy of nbytes: the known-size reference that
+    scales rocprofv3's FETCH_SIZE and WRITE_SIZE counters to HBM bytes."""
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("copied %d bytes" % nbytes)
+
+
+if __name__ == "__main__":
+    copy_probe()
