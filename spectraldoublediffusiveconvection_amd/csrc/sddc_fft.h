@@ -299,4 +299,105 @@ __device__ __forceinline__ void ana_unpack(const cplx<R>* __restrict__ Z, int L,
   c2 = e.x * yr + e.y * yi;
 }
 
+// ---------------------------------------------------------------------------
+// Wavefront-local FFT: one wavefront runs every pass of one transform in its
+// own LDS buffers.  LDS instructions of a wavefront execute in order, so a
+// pass only needs its own LDS writes complete (lgkmcnt(0)) before the next
+// pass reads them; no workgroup barrier.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_lds_sync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt not waited on
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Iterations are processed in chunks of U: every LDS read of a chunk (data
+// and twiddles, indices clamped in range) is issued before any is used, so
+// one wavefront keeps several reads in flight; only stores are predicated.
+template <typename R, int RADIX>
+__device__ void fft_pass_w(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, int L,
+                           int Ns, const cplx<R>* __restrict__ tw, bool inv, int lane) {
+  constexpr int U = RADIX >= 5 ? 2 : (RADIX == 4 ? 3 : 4);
+  const int nb = L / RADIX;
+  const int st = L / (Ns * RADIX);
+  const float rns = 1.0f / (float)Ns;
+  for (int j0 = lane; j0 < nb; j0 += 64 * U) {
+    cplx<R> v[U][RADIX], w[U][RADIX];
+    int base[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int jr = j0 + 64 * u;
+      const int j = jr < nb ? jr : nb - 1;
+      const int jq = qdiv(j, Ns, rns);
+      const int k = j - jq * Ns;
+      base[u] = jq * Ns * RADIX + k;
+#pragma unroll
+      for (int s = 0; s < RADIX; ++s) v[u][s] = in[j + s * nb];
+#pragma unroll
+      for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, s * k * st, inv);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int s = 1; s < RADIX; ++s) v[u][s] = cmul(v[u][s], w[u][s]);
+      if (RADIX == 2) dft2(v[u]);
+      if (RADIX == 3) dft3(v[u], inv);
+      if (RADIX == 4) dft4(v[u], inv);
+      if (RADIX == 5) dft5(v[u], inv);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j0 + 64 * u < nb)
+#pragma unroll
+        for (int s = 0; s < RADIX; ++s) out[base[u] + s * Ns] = v[u][s];
+  }
+}
+
+template <typename R>
+__device__ void fft_pass_any_w(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, int L,
+                               int Ns, int r, const cplx<R>* __restrict__ tw, bool inv, int lane) {
+  const int nb = L / r;
+  const int st = L / (Ns * r);
+  for (int w = lane; w < nb * r; w += 64) {
+    const int s = w % r;
+    const int j = w / r;
+    const int k = j % Ns;
+    cplx<R> acc = {(R)0, (R)0};
+    for (int t = 0; t < r; ++t) {
+      cplx<R> x = in[j + t * nb];
+      if (t) x = cmul(x, tw_get(tw, t * k * st, inv));
+      acc = cadd(acc, cmul(x, tw_get(tw, ((s * t) % r) * nb, inv)));
+    }
+    out[(j / Ns) * Ns * r + k + s * Ns] = acc;
+  }
+}
+
+// All passes of one transform by one wavefront; returns the buffer holding
+// the result (a or b).
+template <typename R>
+__device__ cplx<R>* fft_run_w(cplx<R>* a, cplx<R>* b, const FftShape& sh,
+                              const cplx<R>* __restrict__ tw, bool inv, int lane) {
+  int Ns = 1;
+  for (int p = 0; p < sh.npass; ++p) {
+    const int r = sh.radix[p];
+    if (r == 4)
+      fft_pass_w<R, 4>(a, b, sh.L, Ns, tw, inv, lane);
+    else if (r == 2)
+      fft_pass_w<R, 2>(a, b, sh.L, Ns, tw, inv, lane);
+    else if (r == 3)
+      fft_pass_w<R, 3>(a, b, sh.L, Ns, tw, inv, lane);
+    else if (r == 5)
+      fft_pass_w<R, 5>(a, b, sh.L, Ns, tw, inv, lane);
+    else
+      fft_pass_any_w<R>(a, b, sh.L, Ns, r, tw, inv, lane);
+    wave_lds_sync();
+    cplx<R>* t = a;
+    a = b;
+    b = t;
+    Ns *= r;
+  }
+  return a;
+}
+
 }  // namespace sddc

@@ -88,7 +88,7 @@ struct SpectralArgs {
 };
 
 template <typename R, int NRP>
-__global__ __launch_bounds__(256) void k_spectral(SpectralArgs a) {
+__global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
   constexpr int NT = NRP / 16;
   constexpr int NK = NRP / 4;
   __shared__ double sPsi[kTile + 2][NRP];   // modes k0-1 .. k0+16
@@ -104,11 +104,35 @@ __global__ __launch_bounds__(256) void k_spectral(SpectralArgs a) {
   const double* Xb = a.X + (size_t)b * 3 * N;
   const double* Pb = a.P + (size_t)b * N;
 
-  // all tile loads of a thread are issued before any is used
-  constexpr int kLoadIters = ((kTile + 2) * NRP + 255) / 256;
+  // Wavefront c runs contraction c over all row tiles:
+  //  0 Dr psi, 1 D2r2 psi, 2 D2s psi (float64 always: linear term), 3 Dr T, 4 Dr S.
+  // Its operator tiles are loaded first, all in flight together with the
+  // state tiles below (every global load of the kernel is issued before any
+  // is waited for).
+  const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  const bool f64 = (c == 2) || std::is_same<R, double>::value;
+  const int slot = (c == 1) ? 1 : (c == 2 ? 2 : 0);
+  double a64[NT][NK];
+  float a32[NT][NK];
+  if (f64) {
+    const double* At = a.A64 + (size_t)slot * NT * NK * 64 + lane;
+#pragma unroll
+    for (int it = 0; it < NT; ++it)
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) a64[it][kk] = At[(it * NK + kk) * 64];
+  } else {
+    const float* At = a.A32 + (size_t)slot * NT * NK * 64 + lane;
+#pragma unroll
+    for (int it = 0; it < NT; ++it)
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) a32[it][kk] = At[(it * NK + kk) * 64];
+  }
+
+  constexpr int kLoadIters = ((kTile + 2) * NRP + 319) / 320;
 #pragma unroll
   for (int it = 0; it < kLoadIters; ++it) {
-    const int idx = threadIdx.x + 256 * it;
+    const int idx = threadIdx.x + 320 * it;
     if (idx < (kTile + 2) * NRP) {
       const int t = idx / NRP, i = idx - t * NRP;
       const int k = k0 - 1 + t;
@@ -122,45 +146,38 @@ __global__ __launch_bounds__(256) void k_spectral(SpectralArgs a) {
       }
     }
   }
-  for (int e = threadIdx.x; e < V_COUNT * NRP; e += blockDim.x) sV[e] = a.vecs[e];
+  constexpr int kVecIters = (V_COUNT * NRP + 319) / 320;
+#pragma unroll
+  for (int it = 0; it < kVecIters; ++it) {
+    const int e = threadIdx.x + 320 * it;
+    if (e < V_COUNT * NRP) sV[e] = a.vecs[e];
+  }
   __syncthreads();
 
-  // Five contractions (nr x nr) @ (nr x 16 modes).  Both MFMA shapes take
-  // A[i][k] from lane i + 16k and B[k][j] from lane j + 16k; the output
-  // layouts differ between float64 and float32 (see the stores below).
-  //  0 Dr psi, 1 D2r2 psi, 2 D2s psi (float64 always: linear term), 3 Dr T, 4 Dr S
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int col = lane & 15, kq = lane >> 4;
-  for (int task = wave; task < 5 * NT; task += 4) {
-    const int c = task / NT, it = task - c * NT;
+  // (nr x nr) @ (nr x 16 modes) on MFMA.  Both shapes take A[i][k] from lane
+  // i + 16k and B[k][j] from lane j + 16k; their output layouts differ.
+  {
     const double* src = (c <= 2) ? &sPsi[1][0] : (c == 3 ? &sT[0][0] : &sS[0][0]);
     const double* srow = src + col * NRP + kq;
-    if (c == 2 || std::is_same<R, double>::value) {
-      const int slot = (c == 1) ? 1 : (c == 2 ? 2 : 0);
-      const double* At = a.A64 + ((size_t)(slot * NT + it) * NK) * 64 + lane;
-      double av[NK];
 #pragma unroll
-      for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];   // all operand loads in flight
-      v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int it = 0; it < NT; ++it) {
+      if (f64) {
+        v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int kk = 0; kk < NK; ++kk)
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], srow[4 * kk], acc, 0, 0, 0);
-      // float64 16x16x4 output: lane l, register v holds row l/16 + 4v
+        for (int kk = 0; kk < NK; ++kk)
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a64[it][kk], srow[4 * kk], acc, 0, 0, 0);
+        // float64 16x16x4 output: lane l, register v holds row l/16 + 4v
 #pragma unroll
-      for (int v = 0; v < 4; ++v) sC[c][col][16 * it + kq + 4 * v] = acc[v];
-    } else {
-      const int slot = (c == 1) ? 1 : 0;
-      const float* At = a.A32 + ((size_t)(slot * NT + it) * NK) * 64 + lane;
-      float av[NK];
+        for (int v = 0; v < 4; ++v) sC[c][col][16 * it + kq + 4 * v] = acc[v];
+      } else {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];
-      v4f acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kk = 0; kk < NK; ++kk)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[it][kk], (float)srow[4 * kk], acc, 0, 0, 0);
+        // float32 16x16x4 output: lane l, register v holds row 4*(l/16) + v
 #pragma unroll
-      for (int kk = 0; kk < NK; ++kk)
-        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], (float)srow[4 * kk], acc, 0, 0, 0);
-      // float32 16x16x4 output: lane l, register v holds row 4*(l/16) + v
-#pragma unroll
-      for (int v = 0; v < 4; ++v) sC[c][col][16 * it + 4 * kq + v] = (double)acc[v];
+        for (int v = 0; v < 4; ++v) sC[c][col][16 * it + 4 * kq + v] = (double)acc[v];
+      }
     }
   }
   __syncthreads();
@@ -271,10 +288,10 @@ template <typename R>
 static int launch_spectral_r(const Plan& p, const SpectralArgs& a, int B, hipStream_t s) {
   const dim3 grid((p.K + kTile - 1) / kTile, B);
   switch (p.NRP) {
-    case 16: hipLaunchKernelGGL((k_spectral<R, 16>), grid, dim3(256), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((k_spectral<R, 32>), grid, dim3(256), 0, s, a); break;
-    case 48: hipLaunchKernelGGL((k_spectral<R, 48>), grid, dim3(256), 0, s, a); break;
-    case 64: hipLaunchKernelGGL((k_spectral<R, 64>), grid, dim3(256), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((k_spectral<R, 16>), grid, dim3(320), 0, s, a); break;
+    case 32: hipLaunchKernelGGL((k_spectral<R, 32>), grid, dim3(320), 0, s, a); break;
+    case 48: hipLaunchKernelGGL((k_spectral<R, 48>), grid, dim3(320), 0, s, a); break;
+    case 64: hipLaunchKernelGGL((k_spectral<R, 64>), grid, dim3(320), 0, s, a); break;
     default: return fail(SDDC_ERR_UNSUPPORTED, "unsupported radial size");
   }
   return hip_check(hipGetLastError(), "k_spectral");
@@ -320,7 +337,7 @@ struct RhsArgs {
 
 template <typename R, int NRP>
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
-  constexpr int NT = NRP / 16;
+  constexpr int NT = NRP / 16;   // <= 4: one row tile per wavefront
   constexpr int NK = NRP / 4;
   __shared__ R sQ[kTile][NRP];
   __shared__ R sR[kTile][NRP];
@@ -332,82 +349,105 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
   const size_t nK = (size_t)nr * K;
   const double* Lb = a.L ? a.L + (size_t)b * 3 * N : nullptr;
   double* Gb = a.G + (size_t)b * 3 * N;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  const bool mfma = a.have_nonlin && wave < NT;
+  // Every global load is issued before any is waited for: the Dr row tile of
+  // this wavefront, the linear parts L of this thread's outputs, the product
+  // tiles.
+  double a64[NK];
+  float a32[NK];
+  if (mfma) {
+    if (std::is_same<R, double>::value) {
+      const double* At = a.A64 + ((size_t)wave * NK) * 64 + lane;
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) a64[kk] = At[kk * 64];
+    } else {
+      const float* At = a.A32 + ((size_t)wave * NK) * 64 + lane;
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) a32[kk] = At[kk * 64];
+    }
+  }
+  constexpr int kOut = (kTile * NRP + 255) / 256;
+  double lv[kOut][3];
+#pragma unroll
+  for (int j = 0; j < kOut; ++j) {
+    const int idx = threadIdx.x + 256 * j;
+    const int rl = idx / nr, i = idx - rl * nr;
+    const bool ok = Lb && idx < kTile * nr && k0 + rl < K;
+    const size_t o = ok ? (size_t)(k0 + rl) * nr + i : 0;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) lv[j][f] = ok ? Lb[f * N + o] : 0.0;
+  }
   if (a.have_nonlin) {
     const R* Fb = reinterpret_cast<const R*>(a.F) + (size_t)b * 4 * nK;
     constexpr int kIters = (NRP * kTile + 255) / 256;
 #pragma unroll
     for (int it = 0; it < kIters; ++it) {
       const int idx = threadIdx.x + 256 * it;
-      if (idx >= NRP * kTile) break;
-      const int i = idx / kTile, rl = idx - i * kTile;
-      const int r = k0 + rl, m = r + 1;
-      const bool okm = (i < nr && m < K), okr = (i < nr && r < K);
-      const size_t row = (size_t)i * K;
-      sQ[rl][i] = okm ? Fb[row + m] : (R)0;
-      sR[rl][i] = okm ? Fb[nK + row + m] : (R)0;
-      sTT[rl][i] = okr ? Fb[2 * nK + row + r] : (R)0;
-      sSS[rl][i] = okr ? Fb[3 * nK + row + r] : (R)0;
+      if (idx < NRP * kTile) {
+        const int i = idx / kTile, rl = idx - i * kTile;
+        const int r = k0 + rl, m = r + 1;
+        const bool okm = (i < nr && m < K), okr = (i < nr && r < K);
+        const size_t row = (size_t)i * K;
+        sQ[rl][i] = okm ? Fb[row + m] : (R)0;
+        sR[rl][i] = okm ? Fb[nK + row + m] : (R)0;
+        sTT[rl][i] = okr ? Fb[2 * nK + row + r] : (R)0;
+        sSS[rl][i] = okr ? Fb[3 * nK + row + r] : (R)0;
+      }
     }
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int col = lane & 15, kq = lane >> 4;
-    for (int it = wave; it < NT; it += 4) {
+    if (mfma) {
       const R* srow = &sQ[col][kq];
       if (std::is_same<R, double>::value) {
-        const double* At = a.A64 + ((size_t)it * NK) * 64 + lane;
-        double av[NK];
-#pragma unroll
-        for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];
         v4d acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int kk = 0; kk < NK; ++kk)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kk], (double)srow[4 * kk], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a64[kk], (double)srow[4 * kk], acc, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sDQ[col][16 * it + kq + 4 * v] = (R)acc[v];
+        for (int v = 0; v < 4; ++v) sDQ[col][16 * wave + kq + 4 * v] = (R)acc[v];
       } else {
-        const float* At = a.A32 + ((size_t)it * NK) * 64 + lane;
-        float av[NK];
-#pragma unroll
-        for (int kk = 0; kk < NK; ++kk) av[kk] = At[kk * 64];
         v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kk = 0; kk < NK; ++kk)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[kk], (float)srow[4 * kk], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[kk], (float)srow[4 * kk], acc, 0, 0, 0);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sDQ[col][16 * it + 4 * kq + v] = (R)acc[v];
+        for (int v = 0; v < 4; ++v) sDQ[col][16 * wave + 4 * kq + v] = (R)acc[v];
       }
     }
     __syncthreads();
   }
   const double sc = a.scale;
-  for (int idx = threadIdx.x; idx < kTile * nr; idx += blockDim.x) {
+#pragma unroll
+  for (int j = 0; j < kOut; ++j) {
+    const int idx = threadIdx.x + 256 * j;
     const int rl = idx / nr, i = idx - rl * nr;
     const int r = k0 + rl;
-    if (r >= K) continue;
+    if (idx >= kTile * nr || r >= K) continue;
     const size_t o = (size_t)r * nr + i;
     if (a.have_nonlin) {
       // code row r of the psi term is sinusoid index r+1; row K-1 is zero
       const double Fpsi = (r <= K - 2) ? (double)(sDQ[rl][i] - sR[rl][i]) : 0.0;
       const double Ft = (double)sTT[rl][i], Fs = (double)sSS[rl][i];
       if (Lb) {
-        Gb[o] = sc * Fpsi + Lb[o];
-        Gb[N + o] = sc * Ft + Lb[N + o];
-        Gb[2 * N + o] = sc * Fs + Lb[2 * N + o];
+        Gb[o] = sc * Fpsi + lv[j][0];
+        Gb[N + o] = sc * Ft + lv[j][1];
+        Gb[2 * N + o] = sc * Fs + lv[j][2];
       } else {
         Gb[o] = sc * Fpsi;
         Gb[N + o] = sc * Ft;
         Gb[2 * N + o] = sc * Fs;
       }
     } else {
-      Gb[o] = Lb[o];
-      Gb[N + o] = Lb[N + o];
-      Gb[2 * N + o] = Lb[2 * N + o];
+      Gb[o] = lv[j][0];
+      Gb[N + o] = lv[j][1];
+      Gb[2 * N + o] = lv[j][2];
     }
   }
 }
 
 template <typename R>
-static int launch_rhs_r(const Plan& p, const RhsArgs& a, int B, hipStream_t s) {
+int launch_rhs_r(const Plan& p, const RhsArgs& a, int B, hipStream_t s) {
   const dim3 grid((p.K + kTile - 1) / kTile, B);
   switch (p.NRP) {
     case 16: hipLaunchKernelGGL((k_rhs<R, 16>), grid, dim3(256), 0, s, a); break;

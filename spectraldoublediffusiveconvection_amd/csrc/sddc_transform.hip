@@ -346,6 +346,262 @@ __global__ __launch_bounds__(1024) void k_nonlinear(FftShape sh, const cplx<R>* 
   project_and_store<R>(Z, L, K, rot, F, b, i, nr, inv);
 }
 
+// ---------------------------------------------------------------------------
+// F(X, X) with one wavefront per transform.  A workgroup of five wavefronts
+// handles one (radial row i, instance b).  Wavefront p stages the two series
+// of field pair p in its own LDS buffers, scales each by a power of two
+// (wavefront DPP maximum of exponents), packs them and runs the inverse FFT
+// with no workgroup barrier between passes.  After one workgroup barrier,
+// wavefront 0 forms the sine products (JT*omega, kDpsi*omega + Dpsi*komega)
+// and wavefront 1 the cosine products (T and S), each in a free buffer; each
+// runs its forward FFT and projects its pair onto the K modes.
+// ---------------------------------------------------------------------------
+// Scaled synthesis coefficient a''_k from a staged row c (mode factors
+// already applied): cosine index k, or sine index m = L-k (see syn_coef).
+template <typename R>
+__device__ __forceinline__ R stg_coef(const R* c, int K, int L, int k, bool is_sin) {
+  if (!is_sin) return k < K ? (k == 0 ? c[0] : (R)0.5 * c[k]) : (R)0;
+  const int m = L - k;
+  return (k == 0 || m >= K) ? (R)0 : (R)0.5 * c[m];
+}
+
+template <typename R>
+__global__ __launch_bounds__(320) void k_nonlinear_w(FftShape sh, const cplx<R>* __restrict__ tw_g,
+                                                     const cplx<R>* __restrict__ rot_g, int K,
+                                                     int nr, const R* __restrict__ slabs,
+                                                     R* __restrict__ F) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ R sinv[10];   // unscaling factors of the nine fields
+  __shared__ int sres[5];  // buffer (0 or 1) holding the grid values of pair p
+  using C = cplx<R>;
+  const int L = sh.L;
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* rot = tw + L;
+  C* buf = rot + L;   // [5 pairs][2 buffers][L]
+  // Global loads are issued together (unrolled, predicated) before their
+  // values are used: a loop that waits for each load costs one memory
+  // latency per iteration.
+  {
+    constexpr int NT = 5;   // covers L <= 1600 with 320 threads
+    C t[NT], r[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int kr = threadIdx.x + 320 * j;
+      const int k = kr < L ? kr : L - 1;   // clamped: loads unconditional
+      t[j] = tw_g[k];
+      r[j] = rot_g[k];
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int k = threadIdx.x + 320 * j;
+      if (k < L) {
+        tw[k] = t[j];
+        rot[k] = r[j];
+      }
+    }
+    for (int k = threadIdx.x + 320 * NT; k < L; k += blockDim.x) {
+      tw[k] = tw_g[k];
+      rot[k] = rot_g[k];
+    }
+  }
+  __syncthreads();
+  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int i = blockIdx.x, b = blockIdx.y;
+  const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
+  {
+    C* A = buf + (size_t)p * 2 * L;
+    C* Bb = A + L;
+    const R *u, *v = nullptr;
+    bool us, vs = false;
+    int um, vm = 0;
+    field_src(sg, 2 * p, u, us, um);
+    if (p < 4) field_src(sg, 2 * p + 1, v, vs, vm);
+    R* cu = reinterpret_cast<R*>(Bb);   // staged rows; Bb is free until the FFT
+    R* cv = cu + K;
+    int eu = -1000, ev = -1000;
+    auto stage = [&](int k, R a, R c) {
+      if (um == 1) a = (R)k * a;
+      if (um == 2) a = (R)(-k) * a;
+      if (vm == 1) c = (R)k * c;
+      if (vm == 2) c = (R)(-k) * c;
+      cu[k] = a;
+      cv[k] = c;
+      if (!(us && k == 0)) eu = max(eu, bexp(a));
+      if (!(vs && k == 0)) ev = max(ev, bexp(c));
+    };
+    constexpr int NJ = 16;   // K <= 1024 in one batch of loads
+    R ra[NJ], rc[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int kr = lane + 64 * j;
+      const int k = kr < K ? kr : K - 1;   // clamped: loads unconditional
+      ra[j] = u[k];
+      rc[j] = v ? v[k] : (R)0;
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int k = lane + 64 * j;
+      if (k < K) stage(k, ra[j], rc[j]);
+    }
+    for (int k = lane + 64 * NJ; k < K; k += 64) stage(k, u[k], v ? v[k] : (R)0);
+    eu = __ockl_wfred_max_i32(eu);
+    ev = __ockl_wfred_max_i32(ev);
+    const R su = pow2_scale_e<R>(eu), sv = pow2_scale_e<R>(ev);
+    if (lane == 0) {
+      sinv[2 * p] = (R)1 / su;
+      if (p < 4) sinv[2 * p + 1] = (R)1 / sv;
+    }
+    wave_lds_sync();
+    {
+      const R* __restrict__ ccu = cu;
+      const R* __restrict__ ccv = cv;
+      C* __restrict__ WA = A;
+      constexpr int U = 4;
+      for (int k0 = lane; k0 < L; k0 += 64 * U) {
+        R au[U], av[U], aum[U], avm[U];
+        C rr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int kr = k0 + 64 * u;
+          const int k = kr < L ? kr : L - 1;
+          au[u] = su * stg_coef(ccu, K, L, k, us);
+          av[u] = sv * stg_coef(ccv, K, L, k, vs);
+          aum[u] = k ? su * stg_coef(ccu, K, L, L - k, us) : (R)0;
+          avm[u] = k ? sv * stg_coef(ccv, K, L, L - k, vs) : (R)0;
+          rr[u] = rot[k];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (k0 + 64 * u < L) WA[k0 + 64 * u] = cmul(C{au[u] + avm[u], av[u] - aum[u]}, rr[u]);
+      }
+    }
+    wave_lds_sync();
+    C* res = fft_run_w<R>(A, Bb, sh, tw, true, lane);
+    if (lane == 0) sres[p] = (res == A) ? 0 : 1;
+  }
+  __syncthreads();
+  if (p >= 2) return;
+
+  C* G[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) G[q] = buf + (size_t)q * 2 * L + (size_t)sres[q] * L;
+  R is[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) is[f] = sinv[f];
+  C* O = buf + (size_t)p * 2 * L + (size_t)(1 - sres[p]) * L;              // free buffer of pair p
+  C* O2 = buf + (size_t)(p + 2) * 2 * L + (size_t)(1 - sres[p + 2]) * L;   // free buffer of pair p+2
+  int e0 = -1000, e1 = -1000;
+  {
+    constexpr int U = 4;
+    const C* __restrict__ G0 = G[0];
+    const C* __restrict__ G1 = G[1];
+    const C* __restrict__ G2 = G[2];
+    const C* __restrict__ G3 = G[3];
+    const C* __restrict__ G4 = G[4];
+    C* __restrict__ Ow = O;
+    for (int q0 = lane; q0 < L; q0 += 64 * U) {
+      C w0[U], w1[U], w2[U], w3[U], w4[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int qr = q0 + 64 * u;
+        const int q = qr < L ? qr : L - 1;
+        w0[u] = G0[q];
+        w1[u] = G1[q];
+        w2[u] = G2[q];
+        if (p == 1) {
+          w3[u] = G3[q];
+          w4[u] = G4[q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int q = q0 + 64 * u;
+        const R gs = (R)grid_sign(q, L);
+        const R JT = w0[u].x * is[0], Om = gs * w0[u].y * is[1];
+        const R kOm = w1[u].x * is[2], Dpsi = gs * w1[u].y * is[3];
+        C o;
+        if (p == 0) {
+          // Matrix_Operators.py:791 (Dr applied to JT*omega after projection)
+          const R kDpsi = w2[u].x * is[4];
+          o = {gs * (JT * Om), gs * (kDpsi * Om + Dpsi * kOm)};
+        } else {
+          // Matrix_Operators.py:792-793
+          const R kT = gs * w2[u].y * is[5], DT = w3[u].x * is[6], kC = gs * w3[u].y * is[7],
+                  DC = w4[u].x * is[8];
+          o = {JT * DT - Dpsi * kT, JT * DC - Dpsi * kC};
+        }
+        if (q < L) {
+          Ow[q] = o;
+          e0 = max(e0, bexp(o.x));
+          e1 = max(e1, bexp(o.y));
+        }
+      }
+    }
+  }
+  e0 = __ockl_wfred_max_i32(e0);
+  e1 = __ockl_wfred_max_i32(e1);
+  const R s0 = pow2_scale_e<R>(e0), s1 = pow2_scale_e<R>(e1);
+  {
+    constexpr int U = 4;
+    C* __restrict__ Ow = O;
+    for (int q0 = lane; q0 < L; q0 += 64 * U) {   // the lane's own elements
+      C t[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) t[u] = Ow[(q0 + 64 * u) < L ? q0 + 64 * u : L - 1];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (q0 + 64 * u < L) Ow[q0 + 64 * u] = {t[u].x * s0, t[u].y * s1};
+    }
+  }
+  wave_lds_sync();
+  C* Z = fft_run_w<R>(O, O2, sh, tw, false, lane);
+  const R i0 = (R)1 / s0, i1 = (R)1 / s1, invL = (R)1 / (R)L;
+  R* f0 = F + ((size_t)(b * 4 + 2 * p) * nr + i) * K;
+  R* f1 = F + ((size_t)(b * 4 + 2 * p + 1) * nr + i) * K;
+  {
+    constexpr int U = 4;
+    const C* __restrict__ Zr = Z;
+    for (int k0 = lane; k0 < K; k0 += 64 * U) {
+      C za[U], zb[U], e[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kr = k0 + 64 * u;
+        const int k = kr < K ? kr : K - 1;
+        const int ka = (p == 0) ? L - k : k;   // sine coefficients read index L-k
+        const int kb = (L - ka) % L;
+        za[u] = Zr[ka % L];
+        zb[u] = Zr[kb];
+        e[u] = rot[ka % L];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + 64 * u;
+        // X = (a + conj b)/2, Y = (a - conj b)/(2i); C = Re(conj(e) X), Re(conj(e) Y)
+        const R xr = (R)0.5 * (za[u].x + zb[u].x), xi = (R)0.5 * (za[u].y - zb[u].y);
+        const R yr = (R)0.5 * (za[u].y + zb[u].y), yi = (R)-0.5 * (za[u].x - zb[u].x);
+        const R c1 = e[u].x * xr + e[u].y * xi;
+        const R c2 = e[u].x * yr + e[u].y * yi;
+        R a1, a2;
+        if (p == 0) {   // sine coefficients (k = 0 is zero)
+          a1 = k > 0 ? ((R)2 * (c1 * i0)) * invL : (R)0;
+          a2 = k > 0 ? ((R)2 * (c2 * i1)) * invL : (R)0;
+        } else {        // cosine coefficients
+          a1 = ((R)2 * (c1 * i0)) * invL;
+          a2 = ((R)2 * (c2 * i1)) * invL;
+          if (k == 0) {
+            a1 *= (R)0.5;
+            a2 *= (R)0.5;
+          }
+        }
+        if (k < K) {
+          f0[k] = a1;
+          f1[k] = a2;
+        }
+      }
+    }
+  }
+}
+
 int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current) {
   if (bytes <= 64 * 1024 || bytes <= current) return SDDC_OK;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -366,6 +622,15 @@ template <typename R, bool DFX>
 static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
                        hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
   const size_t lds = 12 * (size_t)p.M * sizeof(cplx<R>);
+  if (!DFX) {
+    if (lds + 64 > 160 * 1024)
+      return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
+    static size_t curw = 0;
+    if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_w<R>, lds, curw)) return rc;
+    hipLaunchKernelGGL((k_nonlinear_w<R>), dim3(p.nr, B), dim3(320), lds, s, p.tabM.shape, tw,
+                       rot, p.K, p.nr, (const R*)slabs, (R*)F);
+    return hip_check(hipGetLastError(), "k_nonlinear_w");
+  }
   if (lds + 16 * 18 * sizeof(int) > 160 * 1024)
     return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
   static size_t cur = 0;
