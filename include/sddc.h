@@ -66,12 +66,28 @@ typedef struct sddc_operators {
 
 typedef struct sddc_plan sddc_plan;
 
+/* Launch-shape choices of a plan (no counterpart in the reference; the
+ * autotuner varies them).  0 leaves a field to the library's default.
+ *   chain_segment    rows per segment of the parity chains of the solves
+ *                    (>= rows per chain: one segment, no join)
+ *   solve_instances  instances per solve workgroup: 1, 2 or 4 */
+typedef struct sddc_tuning {
+  int chain_segment;
+  int solve_instances;
+  int reserved[6];
+} sddc_tuning;
+
 /* Plan lifecycle.  Replaces the operator tuple of Build_Matrix_Operators
  * (Main.py:179-225): uploads the operators once and sizes workspace for
  * max_batch instances.  symmetric=1 applies the equatorial mask of Eq_SYM
  * (Main.py:137-176) to inputs and outputs. */
 int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric,
                      int max_batch, sddc_plan** out);
+/* sddc_plan_create with explicit launch shapes (tuning may be NULL). */
+int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetric,
+                           int max_batch, const sddc_tuning* tuning, sddc_plan** out);
+/* The launch shapes a plan uses (defaults resolved). */
+int sddc_plan_tuning(const sddc_plan* plan, sddc_tuning* out);
 int sddc_plan_reserve(sddc_plan* plan, int max_batch);
 int sddc_plan_destroy(sddc_plan* plan);
 int sddc_plan_info(const sddc_plan* plan, long long* workspace_bytes, int* lds_nonlinear);
@@ -130,8 +146,48 @@ int sddc_diagnostics(sddc_plan* plan, int B, const double* X, double* out, void*
 typedef struct sddc_graph sddc_graph;
 int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
                       double* X, int nsteps, void* stream, sddc_graph** out);
+/* One graph for several size classes: nsteps in-place steps of batch
+ * X[c] (B[c] instances, parameters Ra[c], Ra_s[c]) of every plan c.  Each
+ * class runs on its own forked stream inside the graph, so classes execute
+ * concurrently.  No counterpart in the reference: batches parameter sweeps
+ * over mixed resolutions into one launch. */
+int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
+                            const double* const* Ra, const double* const* Ra_s,
+                            double* const* X, int nsteps, void* stream, sddc_graph** out);
 int sddc_graph_launch(sddc_graph* g, void* stream);
 int sddc_graph_destroy(sddc_graph* g);
+
+/* Matrix-free Newton-Krylov steady solves (_Newton, Main.py:430-554), float64
+ * on the device: restarted GMRES(restart) with CGS2 Gram-Schmidt on a Krylov
+ * basis held by the sddc_krylov object (its only allocation).  One instance
+ * (B = 1) per call; X, b, x are device vectors of 3N values. */
+typedef struct sddc_krylov sddc_krylov;
+typedef struct sddc_krylov_opts {
+  double tol_newton;      /* stop when ||dv|| / ||X|| <= tol_newton (Main.py:518)      */
+  double tol_gmres;       /* Newton: Krylov atol = tol_gmres * ||F|| (Main.py:534)    */
+  double rtol, atol;      /* sddc_gmres: stop at ||b - J x|| <= max(atol, rtol ||b||) */
+  int max_restarts;       /* GMRES cycles (lgmres maxiter, Main.py:534: 250)          */
+  int max_newton;         /* Newton iterations (Main.py:518: 5)                       */
+  int finite_difference;  /* 1: J v by finite differences of the step, 0: PDFX        */
+  int reserved[5];
+} sddc_krylov_opts;
+typedef struct sddc_krylov_info {
+  int converged, iterations, restarts;
+  double residual, rhs_norm;
+} sddc_krylov_info;
+typedef struct sddc_newton_info {
+  int converged, iterations, krylov_iterations;
+  double error;
+} sddc_newton_info;
+int sddc_krylov_create(sddc_plan* plan, int restart, sddc_krylov** out);
+int sddc_krylov_destroy(sddc_krylov* k);
+/* Solve J x = b, J = PDFX at X (Step'(X) - I); x holds the initial guess. */
+int sddc_gmres(sddc_krylov* k, const double* Ra, const double* Ra_s, const double* X,
+               const double* b, double* x, const sddc_krylov_opts* opts, sddc_krylov_info* info,
+               void* stream);
+/* Newton iteration on Step(X) - X = 0 in place (X is overwritten). */
+int sddc_newton(sddc_krylov* k, const double* Ra, const double* Ra_s, double* X,
+                const sddc_krylov_opts* opts, sddc_newton_info* info, void* stream);
 
 /* Launch one stage of the step on the plan's workspace (for timing and
  * profiling; no counterpart in the reference).  stage: 0 suffix sums,

@@ -36,6 +36,29 @@ class SddcOperators(ctypes.Structure):
     ]
 
 
+class SddcTuning(ctypes.Structure):
+    _fields_ = [("chain_segment", ctypes.c_int), ("solve_instances", ctypes.c_int),
+                ("reserved", ctypes.c_int * 6)]
+
+
+class SddcKrylovOpts(ctypes.Structure):
+    _fields_ = [("tol_newton", ctypes.c_double), ("tol_gmres", ctypes.c_double),
+                ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+                ("max_restarts", ctypes.c_int), ("max_newton", ctypes.c_int),
+                ("finite_difference", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+
+
+class SddcKrylovInfo(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("residual", ctypes.c_double),
+                ("rhs_norm", ctypes.c_double)]
+
+
+class SddcNewtonInfo(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("krylov_iterations", ctypes.c_int), ("error", ctypes.c_double)]
+
+
 class SddcError(RuntimeError):
     """A libsddc call failed (HIP error or unsupported size)."""
 
@@ -47,6 +70,9 @@ _V = ctypes.c_void_p
 _I = ctypes.c_int
 _SIGS = {
     "sddc_plan_create": (_I, [ctypes.POINTER(SddcOperators), _I, _I, _I, ctypes.POINTER(_V)]),
+    "sddc_plan_create_tuned": (_I, [ctypes.POINTER(SddcOperators), _I, _I, _I,
+                                    ctypes.POINTER(SddcTuning), ctypes.POINTER(_V)]),
+    "sddc_plan_tuning": (_I, [_V, ctypes.POINTER(SddcTuning)]),
     "sddc_plan_reserve": (_I, [_V, _I]),
     "sddc_plan_destroy": (_I, [_V]),
     "sddc_plan_info": (_I, [_V, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_I)]),
@@ -60,9 +86,18 @@ _SIGS = {
     "sddc_step_linearized": (_I, [_V, _I, _V, _V, _V, _V, _V, _V]),
     "sddc_diagnostics": (_I, [_V, _I, _V, _V, _V]),
     "sddc_graph_create": (_I, [_V, _I, _V, _V, _V, _I, _V, ctypes.POINTER(_V)]),
+    "sddc_graph_create_multi": (_I, [_I, ctypes.POINTER(_V), ctypes.POINTER(_I),
+                                     ctypes.POINTER(_V), ctypes.POINTER(_V), ctypes.POINTER(_V),
+                                     _I, _V, ctypes.POINTER(_V)]),
     "sddc_graph_launch": (_I, [_V, _V]),
     "sddc_graph_destroy": (_I, [_V]),
     "sddc_run_stage": (_I, [_V, _I, _I, _V, _V, _V, _V]),
+    "sddc_krylov_create": (_I, [_V, _I, ctypes.POINTER(_V)]),
+    "sddc_krylov_destroy": (_I, [_V]),
+    "sddc_gmres": (_I, [_V, _V, _V, _V, _V, _V, ctypes.POINTER(SddcKrylovOpts),
+                        ctypes.POINTER(SddcKrylovInfo), _V]),
+    "sddc_newton": (_I, [_V, _V, _V, _V, ctypes.POINTER(SddcKrylovOpts),
+                         ctypes.POINTER(SddcNewtonInfo), _V]),
     "sddc_last_error": (ctypes.c_char_p, []),
     "sddc_version": (ctypes.c_char_p, []),
 }

@@ -1,0 +1,138 @@
+"""Autotuner of the plans' launch shapes.
+
+For one size class and batch size it times HIP-graph-replayed steps under each
+candidate launch shape (rows per chain segment of the solves, instances per
+solve workgroup) and keeps the fastest.  Results are stored in a JSON table
+keyed by size class, precision and batch bucket; ``TuningTable.lookup``
+serves them to batching.Sweep and the serving engine.
+
+    python -m spectraldoublediffusiveconvection_amd.autotune [--out table.json]
+
+tunes the sweep's twelve size classes (N_r in {20,30,40} x N_fm in
+{64,128,256,512}) for a range of batch sizes.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_TABLE = os.path.join(PKG, "tuning_mi355x.json")
+BUCKETS = (1, 4, 16, 64, 256)
+
+
+def bucket(B):
+    """Smallest tuned batch bucket holding B instances."""
+    for b in BUCKETS:
+        if B <= b:
+            return b
+    return BUCKETS[-1]
+
+
+def candidates(N_fm):
+    """Launch shapes tried for a class with N_fm modes (N_fm/2 rows per chain)."""
+    n = N_fm // 2
+    segs = sorted({n, max(8, 4 * math.ceil(math.sqrt(n) / 4)),
+                   max(8, 4 * math.ceil(2 * math.sqrt(n) / 4))})
+    return [{"chain_segment": m, "solve_instances": nb} for m in segs for nb in (1, 2, 4)]
+
+
+def _key(N_fm, N_r, precision, B):
+    return "%d,%d,%s,%d" % (N_fm, N_r, precision, bucket(B))
+
+
+class TuningTable:
+    def __init__(self, path=DEFAULT_TABLE):
+        self.path = path
+        self.entries = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.entries = json.load(f).get("entries", {})
+
+    def lookup(self, key, B, precision):
+        """Tuning for a size-class key (N_fm, N_r, ...) at batch B, or None."""
+        e = self.entries.get(_key(key[0], key[1], precision, B))
+        return dict(e["tuning"]) if e else None
+
+    def put(self, N_fm, N_r, precision, B, tuning, timings):
+        self.entries[_key(N_fm, N_r, precision, B)] = {"tuning": tuning, "timings_us": timings}
+
+    def save(self, path=None):
+        with open(path or self.path, "w") as f:
+            json.dump({"device": "MI355X (gfx950)", "entries": self.entries}, f, indent=1,
+                      sort_keys=True)
+
+
+_OPS = {}
+
+
+def _ops_for(N_fm, N_r, d, dt):
+    from . import operators as ops
+    key = (N_fm, N_r, d, dt)
+    if key not in _OPS:
+        _OPS.clear()
+        _OPS[key] = ops.build(N_fm, N_r, d, dt, 1.0, 1.0 / 15.0)
+    return _OPS[key]
+
+
+def time_shape(N_fm, N_r, B, precision, tuning, steps=10, reps=3, d=0.35, dt=5e-3):
+    """Best µs per batched step under one launch shape (graph replay)."""
+    import torch
+
+    from . import device as dev
+    plan = dev.Plan(_ops_for(N_fm, N_r, d, dt), precision=precision, max_batch=B, tuning=tuning)
+    n3 = 3 * N_fm * (N_r - 1)
+    x = np.random.default_rng(0).standard_normal((B, n3))
+    X = torch.as_tensor(1e-3 * x / np.linalg.norm(x, axis=1, keepdims=True), device="cuda")
+    g = plan.graph(X, np.linspace(2500.0, 3000.0, B), 500.0, nsteps=steps)
+    g.launch()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g.launch()
+        torch.cuda.synchronize()
+        best = min(best, (time.perf_counter() - t0) / steps * 1e6)
+    g.close()
+    plan.close()
+    return best
+
+
+def tune_class(N_fm, N_r, B, precision="mixed", steps=10):
+    """Best launch shape for one class and batch size: (tuning, {shape: µs})."""
+    timings = {}
+    best, best_t = None, float("inf")
+    for t in candidates(N_fm):
+        us = time_shape(N_fm, N_r, B, precision, t, steps=steps)
+        timings["seg%d_nb%d" % (t["chain_segment"], t["solve_instances"])] = round(us, 2)
+        if us < best_t:
+            best, best_t = t, us
+    return best, timings
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--precision", default="mixed")
+    ap.add_argument("--N_r", type=int, nargs="*", default=[20, 30, 40])
+    ap.add_argument("--N_fm", type=int, nargs="*", default=[64, 128, 256, 512])
+    ap.add_argument("--batches", type=int, nargs="*", default=list(BUCKETS))
+    args = ap.parse_args()
+    table = TuningTable(args.out)
+    for N_r in args.N_r:
+        for N_fm in args.N_fm:
+            for B in args.batches:
+                t, tm = tune_class(N_fm, N_r, B, args.precision)
+                table.put(N_fm, N_r, args.precision, B, t, tm)
+                print("N_r=%d N_fm=%d B=%d -> %s  %s" % (N_r, N_fm, B, t, tm), flush=True)
+                table.save()
+    table.save()
+
+
+if __name__ == "__main__":
+    main()

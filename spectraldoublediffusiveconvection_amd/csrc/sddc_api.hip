@@ -6,16 +6,17 @@
 #include <map>
 #include <mutex>
 
+#include <vector>
+
 #include "sddc_internal.h"
 
-struct sddc_plan {
-  sddc::Plan p;
-};
 
 struct sddc_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t own = nullptr;
+  std::vector<hipStream_t> side;   // per-class capture streams (multi-class graphs)
+  std::vector<hipEvent_t> events;
 };
 
 namespace sddc {
@@ -181,6 +182,19 @@ const char* sddc_version(void) { return "sddc 0.1 gfx950"; }
 
 int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric, int max_batch,
                      sddc_plan** out) {
+  return sddc_plan_create_tuned(ops, precision, symmetric, max_batch, nullptr, out);
+}
+
+int sddc_plan_tuning(const sddc_plan* plan, sddc_tuning* out) {
+  if (!plan || !out) return fail(SDDC_ERR_INVALID, "null argument");
+  *out = sddc_tuning{};
+  out->chain_segment = plan->p.seg_m;
+  out->solve_instances = plan->p.tune_solve_nbi;
+  return SDDC_OK;
+}
+
+int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetric, int max_batch,
+                           const sddc_tuning* tuning, sddc_plan** out) {
   if (!ops || !out) return fail(SDDC_ERR_INVALID, "null argument");
   *out = nullptr;
   if (ops->N_fm % 2 != 0) {
@@ -246,7 +260,15 @@ int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric, in
   }
   if ((rc = upload(&p.vecs, v.data(), v.size()))) return bail(rc);
   p.NRS = ((nr + 7) / 8) * 8;
-  if ((rc = build_chain(p, ops, max_batch > 0 ? max_batch : 1))) return bail(rc);
+  if (tuning && tuning->solve_instances != 0 && tuning->solve_instances != 1 &&
+      tuning->solve_instances != 2 && tuning->solve_instances != 4)
+    return bail(fail(SDDC_ERR_INVALID, "solve_instances must be 0, 1, 2 or 4"));
+  if (tuning && tuning->chain_segment < 0)
+    return bail(fail(SDDC_ERR_INVALID, "chain_segment must be >= 0"));
+  p.tune_solve_nbi = tuning ? tuning->solve_instances : 0;
+  if ((rc = build_chain(p, ops, max_batch > 0 ? max_batch : 1,
+                        tuning ? tuning->chain_segment : 0)))
+    return bail(rc);
   if ((rc = build_tables(p.tabM, p.M))) return bail(rc);
   if ((rc = build_tables(p.tabKE, 3 * p.K))) return bail(rc);
   if ((rc = alloc_ws(p, max_batch > 0 ? max_batch : 1))) return bail(rc);
@@ -524,6 +546,60 @@ int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra
   return SDDC_OK;
 }
 
+int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
+                            const double* const* Ra, const double* const* Ra_s,
+                            double* const* X, int nsteps, void* stream, sddc_graph** out) {
+  if (!plans || !B || !Ra || !Ra_s || !X || !out || nplans < 1)
+    return fail(SDDC_ERR_INVALID, "null argument or no plans");
+  if (nsteps < 1) return fail(SDDC_ERR_INVALID, "nsteps must be positive");
+  for (int c = 0; c < nplans; ++c) {
+    if (!plans[c]) return fail(SDDC_ERR_INVALID, "null plan");
+    if (int rc = check_batch(plans[c]->p, B[c])) return rc;
+  }
+  sddc_graph* g = new sddc_graph();
+  auto bail = [&](int rc) {
+    sddc_graph_destroy(g);
+    return rc;
+  };
+  hipStream_t s = (hipStream_t)stream;
+  if (!s) {
+    if (hipStreamCreateWithFlags(&g->own, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(SDDC_ERR_HIP, "stream creation failed"));
+    s = g->own;
+  }
+  g->side.assign(nplans, nullptr);
+  g->events.assign(nplans + 1, nullptr);
+  for (int c = 0; c < nplans; ++c)
+    if (hipStreamCreateWithFlags(&g->side[c], hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(SDDC_ERR_HIP, "stream creation failed"));
+  for (auto& e : g->events)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(SDDC_ERR_HIP, "event creation failed"));
+  // one eager step per class so that every lazily set kernel attribute is in place
+  int rc = SDDC_OK;
+  for (int c = 0; c < nplans && !rc; ++c)
+    rc = sddc_step(plans[c], B[c], Ra[c], Ra_s[c], X[c], plans[c]->p.Xb, 1, 0, s);
+  if (!rc) rc = hip_check(hipStreamSynchronize(s), "warm-up");
+  if (rc) return bail(rc);
+  // fork: every class stream waits for the capture origin, runs its steps,
+  // and the origin waits for all of them
+  if ((rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture")))
+    return bail(rc);
+  int rc2 = hip_check(hipEventRecord(g->events[nplans], s), "fork");
+  for (int c = 0; c < nplans && !rc2; ++c) {
+    rc2 = hip_check(hipStreamWaitEvent(g->side[c], g->events[nplans], 0), "fork wait");
+    if (!rc2) rc2 = sddc_step(plans[c], B[c], Ra[c], Ra_s[c], X[c], X[c], nsteps, 0, g->side[c]);
+    if (!rc2) rc2 = hip_check(hipEventRecord(g->events[c], g->side[c]), "join");
+    if (!rc2) rc2 = hip_check(hipStreamWaitEvent(s, g->events[c], 0), "join wait");
+  }
+  hipError_t e = hipStreamEndCapture(s, &g->graph);
+  rc = rc2 ? rc2 : hip_check(e, "end capture");
+  if (!rc) rc = hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "instantiate");
+  if (rc) return bail(rc);
+  *out = g;
+  return SDDC_OK;
+}
+
 int sddc_graph_launch(sddc_graph* g, void* stream) {
   if (!g) return fail(SDDC_ERR_INVALID, "null graph");
   hipStream_t s = stream ? (hipStream_t)stream : g->own;
@@ -535,6 +611,10 @@ int sddc_graph_destroy(sddc_graph* g) {
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->own) (void)hipStreamDestroy(g->own);
+  for (hipStream_t st : g->side)
+    if (st) (void)hipStreamDestroy(st);
+  for (hipEvent_t ev : g->events)
+    if (ev) (void)hipEventDestroy(ev);
   delete g;
   return SDDC_OK;
 }

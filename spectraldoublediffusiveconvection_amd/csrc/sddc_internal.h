@@ -135,7 +135,7 @@ int launch_rhs(const Plan& p, int B, const void* F, const double* L, int have_no
                double* G, hipStream_t s);
 int launch_solve(const Plan& p, int B, const double* G, double* Xout, double* Pout,
                  int fields_mask, int mask_out, hipStream_t s);
-int build_chain(Plan& p, const sddc_operators* ops, int max_batch);
+int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment);
 void free_chain(Plan& p);
 int launch_mask(const Plan& p, int B, const double* X, double* out, hipStream_t s);
 int launch_transform(const TableSet& t, int kind, int rows, int len_in, int n_out,
@@ -148,3 +148,9 @@ int nonlinear_lds_bytes(const Plan& p, int precision, int dfx);
 int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current);
 
 }  // namespace sddc
+
+// The opaque handle of include/sddc.h.
+struct sddc_plan {
+  sddc::Plan p;
+};
+

@@ -702,11 +702,11 @@ static int upload_vec(T** dst, const std::vector<T>& v) {
   return SDDC_OK;
 }
 
-int build_chain(Plan& p, const sddc_operators* ops, int max_batch) {
+int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment) {
   const int K = p.K, nr = p.nr, NRS = p.NRS, NRP = p.NRP;
   const size_t BLK = (size_t)NRP * NRS, NN = (size_t)nr * nr;
   const int n = K / 2;   // rows per chain (K is even)
-  int m = default_segment(n, max_batch);
+  int m = segment > 0 ? segment : default_segment(n, max_batch);
   if (m > n) m = n;
   p.seg_m = m;
   p.nseg = (n + m - 1) / m;

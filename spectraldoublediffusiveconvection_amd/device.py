@@ -38,7 +38,8 @@ class Plan:
     """Operators of one size class uploaded once (Build_Matrix_Operators,
     Main.py:179-225) plus workspace for up to ``max_batch`` instances."""
 
-    def __init__(self, ops: _ops.OperatorSet, symmetric=False, precision="fp64", max_batch=1):
+    def __init__(self, ops: _ops.OperatorSet, symmetric=False, precision="fp64", max_batch=1,
+                 tuning=None):
         require_gpu()
         L = _lib.lib()
         self.ops = ops
@@ -61,15 +62,28 @@ class Plan:
                      "IR2_A4", "IR4_A4", "Linv_A4", "Linv_T", "Linv_S"):
             setattr(s, name, arr(name))
         h = ctypes.c_void_p()
-        _lib.check(L.sddc_plan_create(ctypes.byref(s), self.precision, int(self.symmetric),
-                                      int(max_batch), ctypes.byref(h)))
+        tun = None
+        if tuning:
+            tun = _lib.SddcTuning()
+            tun.chain_segment = int(tuning.get("chain_segment", 0))
+            tun.solve_instances = int(tuning.get("solve_instances", 0))
+        _lib.check(L.sddc_plan_create_tuned(ctypes.byref(s), self.precision, int(self.symmetric),
+                                            int(max_batch), ctypes.byref(tun) if tun else None,
+                                            ctypes.byref(h)))
         self._h = h
         self.max_batch = int(max_batch)
         self.device = torch.device("cuda", torch.cuda.current_device())
 
     @classmethod
-    def build(cls, N_fm, N_r, d, dt, Pr, Tau, symmetric=False, precision="fp64", max_batch=1):
-        return cls(_ops.build(N_fm, N_r, d, dt, Pr, Tau), symmetric, precision, max_batch)
+    def build(cls, N_fm, N_r, d, dt, Pr, Tau, symmetric=False, precision="fp64", max_batch=1,
+              tuning=None):
+        return cls(_ops.build(N_fm, N_r, d, dt, Pr, Tau), symmetric, precision, max_batch, tuning)
+
+    def tuning(self):
+        """Launch shapes in use: {'chain_segment': rows, 'solve_instances': n (0 = by batch)}."""
+        t = _lib.SddcTuning()
+        _lib.check(_lib.lib().sddc_plan_tuning(self._h, ctypes.byref(t)))
+        return {"chain_segment": t.chain_segment, "solve_instances": t.solve_instances}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -3,11 +3,13 @@ time stepper and the Newton solver, with every step, linearised step and
 diagnostic computed by libsddc on the GPU.
 
 Signatures and return values follow Main.py (file:line on each function).
-_Newton keeps the reference's Krylov method (scipy's lgmres on the host, the
-same arguments) around the device's step and linearised step; its iterates
-match the reference's up to rounding, and converged states agree to the
-Newton tolerance.  Checkpoints use h5py with the reference's groups when
-h5py is importable, otherwise a NumPy .npz beside the requested name.
+_Newton runs the device Newton-Krylov solver (restarted GMRES in float64 with
+CGS2 Gram-Schmidt, sddc_newton) with the reference's tolerances and limits;
+set KRYLOV = "lgmres" to use scipy's lgmres on the host around the device's
+step and linearised step instead (the reference's own Krylov method).  Both
+converge to the same steady state within the Newton tolerance.  Checkpoints
+use h5py with the reference's groups when h5py is importable, otherwise a
+NumPy .npz beside the requested name.
 """
 from __future__ import annotations
 
@@ -158,11 +160,48 @@ def _Time_Step(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, save_filename='Tim
     return X_new
 
 
+KRYLOV = "device"
+_krylov = {}
+
+
 def _Newton(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt=1, tol_newton=1e-08,
             tol_gmres=1e-04, Krylov_Space_Size=300):
-    """Steady state by Newton iteration on Step(X) - X with lgmres on the
-    linearised step (Main.py:430-554).  Returns (X, Norm, KE, NuT, NuS, True)
-    or (None, None, None, None, None, False)."""
+    """Steady state by Newton iteration on Step(X) - X (Main.py:430-554).
+    Returns (X, Norm, KE, NuT, NuS, True) or (None, None, None, None, None, False)."""
+    if KRYLOV == "lgmres":
+        return _newton_lgmres(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt, tol_newton,
+                              tol_gmres, Krylov_Space_Size)
+    from ..newton import Krylov
+    plan = _reg.step_plan(N_fm, N_r, d, dt, Pr, Tau, symmetric)
+    # one Krylov basis, rebuilt when the plan (an object the registry may have
+    # closed and replaced) or the restart length changes
+    kr = _krylov.get("k")
+    if kr is None or kr.plan is not plan or kr.restart != int(Krylov_Space_Size):
+        if kr is not None:
+            kr.close()
+        kr = _krylov["k"] = Krylov(plan, restart=Krylov_Space_Size)
+    Xd, info = kr.newton(_dev(X), Ra, Ra_s, tol_newton=tol_newton, tol_gmres=tol_gmres,
+                         max_newton=5, max_restarts=250)
+    print('Newton Iterations = %d, Error = %e, Krylov iterations = %d'
+          % (info["iterations"], info["error"], info["krylov_iterations"]), "\n")
+    if not info["converged"]:
+        print('Newton iteration could not converge')
+        return None, None, None, None, None, False
+    ops = plan.ops
+    R, D = ops.R, ops.D
+    nr = N_r - 1
+    N = N_fm * nr
+    X = _host(Xd.view(1, -1))
+    Norm = np.linalg.norm(X, 2)
+    KE = Kinetic_Energy(X, R, D, N_fm, nr, symmetric)
+    NuT = Nusselt(X[N:2 * N], d, R, D, N_fm, nr)
+    NuS = Nusselt(X[2 * N:3 * N], d, R, D, N_fm, nr)
+    return X, Norm, KE, NuT, NuS, True
+
+
+def _newton_lgmres(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt, tol_newton, tol_gmres,
+                   Krylov_Space_Size):
+    """_Newton with scipy's lgmres on the host (the reference's Krylov method)."""
     import scipy.sparse.linalg as spla
     plan = _reg.step_plan(N_fm, N_r, d, dt, Pr, Tau, symmetric)
     ops = plan.ops

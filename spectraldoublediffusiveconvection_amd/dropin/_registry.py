@@ -58,8 +58,18 @@ def _get(key, make):
     _plans[key] = p
     while len(_plans) > _MAX_PLANS:
         _, old = _plans.popitem(last=False)
+        _release(old)
         old.close()
     return p
+
+
+def _release(plan):
+    """Close the Krylov basis of the drop-in Newton solver if it uses plan."""
+    from . import Main
+    kr = Main._krylov.get("k")
+    if kr is not None and kr.plan is plan:
+        kr.close()
+        Main._krylov.clear()
 
 
 def _dummy_geometry(nr):
@@ -131,4 +141,5 @@ def step_plan(N_fm, N_r, d, dt, Pr, Tau, symmetric):
 def clear():
     while _plans:
         _, p = _plans.popitem()
+        _release(p)
         p.close()

@@ -1,0 +1,78 @@
+"""Mixed-resolution sweeps: one HIP graph over several size classes, launch
+shapes, and the autotuner's table.  GPU tests check that the graph of a sweep
+gives the same bits as stepping each class eagerly, and that every instance
+gives the same bits alone (same launch shapes) as in the sweep."""
+import json
+
+import numpy as np
+import pytest
+
+from spectraldoublediffusiveconvection_amd import autotune
+
+
+def test_autotune_buckets_and_candidates(tmp_path):
+    assert [autotune.bucket(b) for b in (1, 2, 4, 5, 16, 17, 300)] == [1, 4, 4, 16, 16, 64, 256]
+    c = autotune.candidates(512)
+    segs = sorted({t["chain_segment"] for t in c})
+    assert segs == [16, 32, 256] and {t["solve_instances"] for t in c} == {1, 2, 4}
+    t = autotune.TuningTable(str(tmp_path / "t.json"))
+    t.put(64, 20, "mixed", 10, {"chain_segment": 8, "solve_instances": 2}, {"x": 1.0})
+    t.save()
+    again = autotune.TuningTable(str(tmp_path / "t.json"))
+    assert again.lookup((64, 20), 12, "mixed") == {"chain_segment": 8, "solve_instances": 2}
+    assert again.lookup((64, 20), 40, "mixed") is None
+    assert json.load(open(tmp_path / "t.json"))["entries"]
+
+
+@pytest.mark.gpu
+def test_sweep_graph_matches_eager_and_alone():
+    import torch
+
+    from spectraldoublediffusiveconvection_amd import batching, device as dev, operators as ops
+    insts = batching.random_sweep(9, seed=3, N_r=(12, 20), N_fm=(16, 64))
+    tuning = {"chain_segment": 8, "solve_instances": 2}
+
+    class Fixed:
+        def lookup(self, key, B, precision):
+            return tuning
+    a = batching.Sweep(insts, precision="fp64", tuning_table=Fixed())
+    b = batching.Sweep(insts, precision="fp64", tuning_table=Fixed())
+    assert len(a.classes) >= 2
+    # the same call boundaries on both sides: each call starts by recomputing
+    # the psi suffix sums, which later steps take from the previous solve
+    g = a.graph(nsteps=5)
+    for _ in range(2):
+        g.launch()
+        b.step(5)
+    sa, sb = a.states(), b.states()
+    for x, y in zip(sa, sb):
+        assert np.array_equal(x, y)
+    g.close()
+    # each instance alone, in its own plan with the same launch shapes
+    for inst, x in zip(insts, sa):
+        p = dev.Plan(ops.build(inst["N_fm"], inst["N_r"], inst["d"], inst["dt"], inst["Pr"],
+                               inst["Tau"]), precision="fp64", tuning=tuning)
+        y = torch.as_tensor(inst["X0"], device="cuda")
+        for _ in range(2):
+            y = p.step(y, inst["Ra"], inst["Ra_s"], nsteps=5)
+        torch.cuda.synchronize()
+        assert np.array_equal(y.cpu().numpy()[0], x)
+        assert p.tuning() == tuning
+
+
+@pytest.mark.gpu
+def test_tuned_plans_agree_with_oracle():
+    """Every launch shape the autotuner may pick gives the oracle's step."""
+    import torch
+
+    from oracle import sddc_oracle as O
+    from spectraldoublediffusiveconvection_amd import device as dev, operators as ops
+    K, Nr = 64, 20
+    o = O.build_operators(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0)
+    X = 1e-3 * np.random.default_rng(8).standard_normal((3, 3 * K * (Nr - 1)))
+    ref = [O.step(x, o, 3000.0, 500.0) for x in X]
+    for t in autotune.candidates(K):
+        p = dev.Plan(ops.build(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0), max_batch=3, tuning=t)
+        y = p.step(torch.as_tensor(X, device="cuda"), 3000.0, 500.0).cpu().numpy()
+        for a, b in zip(y, ref):
+            assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-12, t
