@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--graph-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
     return ap.parse_args()
 
 
@@ -62,6 +66,59 @@ def measured_traffic(kernel, B):
     if not rec or rec.get("workload") != "N_r=40 N_fm=512 B=%d mixed" % B:
         return None
     return rec["traffic_bytes"]
+
+
+def sweep_c4(tuning_table, n=256, graph_steps=10, launches=5):
+    """Mixed-resolution sweep: n instances with N_r in {20,30,40} and N_fm in
+    {64,128,256,512} (seeded), one HIP graph over all size classes."""
+    import torch
+
+    from spectraldoublediffusiveconvection_amd import batching
+    w = WORKLOAD
+    insts = batching.random_sweep(n, seed=2024, Ra=(w["Ra_lo"], w["Ra_hi"]))
+    sw = batching.Sweep(insts, precision="mixed", tuning_table=tuning_table)
+    g = sw.graph(nsteps=graph_steps)
+    g.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        g.launch()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    finite = all(bool(torch.isfinite(c.X).all().item()) for c in sw.classes.values())
+    g.close()
+    return {"instances": n, "size_classes": len(sw.classes), "graph_steps": graph_steps,
+            "instance_steps_per_s": n * graph_steps * launches / el,
+            "ms_per_sweep_step": 1e3 * el / (graph_steps * launches), "states_finite": finite}
+
+
+def serving_run(rate, seconds, tuning_table):
+    """Poisson stream of mixed jobs through the serving engine: 60% time
+    integrations (500 steps, sizes as in the sweep), 30% Newton solves and 10%
+    20-point continuations in Ra_T of steady wide-gap convection (d=2, Pr=10,
+    N_r=16, N_fm=32, started near the steady state)."""
+    import torch
+
+    from spectraldoublediffusiveconvection_amd import device as dev
+    from spectraldoublediffusiveconvection_amd import serving
+    Kw, Nrw, d = 32, 16, 2.0
+    tsp = dev.Plan.build(Kw, Nrw, d, 0.075, 10.0, 1.0, symmetric=True)
+    x0 = np.random.default_rng(0).random(3 * Kw * (Nrw - 1))
+    X = tsp.step(torch.as_tensor(1e-3 * x0 / np.linalg.norm(x0), device="cuda"), 6780.0, 0.0,
+                 nsteps=26667)[0].cpu().numpy()
+    case = dict(N_fm=Kw, N_r=Nrw, d=d, Pr=10.0, Tau=1.0, Ra=6780.0, Ra_s=0.0, X=X,
+                symmetric=True, restart=100, dRa=10.0)
+    w = WORKLOAD
+    jobs = serving.poisson_stream(rate, seconds, seed=7, newton_case=case,
+                                  ra=(w["Ra_lo"], w["Ra_hi"]))
+    eng = serving.Engine(precision="mixed", capacity=64, chunk=25, newton_workers=3,
+                         tuning_table=tuning_table)
+    eng.run(jobs, realtime=True, max_seconds=seconds * 10 + 30)
+    eng.shutdown()
+    st = eng.stats()
+    st.update({"rate_jobs_per_s": rate, "arrival_window_s": seconds, "submitted": len(jobs),
+               "mix": "60% time (500 steps), 30% newton, 10% continuation (20 points)"})
+    return st
 
 
 def cpu_baseline(seconds):
@@ -95,6 +152,7 @@ def main():
         sys.exit(2)
     import torch
 
+    from spectraldoublediffusiveconvection_amd import autotune
     from spectraldoublediffusiveconvection_amd import device as dev
     from spectraldoublediffusiveconvection_amd import operators as ops
     from spectraldoublediffusiveconvection_amd import profiling
@@ -104,9 +162,11 @@ def main():
     op = ops.build(w["N_fm"], w["N_r"], w["d"], w["dt"], w["Pr"], w["Tau"])
     n3 = 3 * w["N_fm"] * (w["N_r"] - 1)
     Ra = np.linspace(w["Ra_lo"], w["Ra_hi"], B)
+    table = autotune.TuningTable()
+    tuning = table.lookup((w["N_fm"], w["N_r"]), B, "mixed")
 
     # ---- batched mixed-precision step, graph-replayed
-    plan = dev.Plan(op, precision="mixed", max_batch=B)
+    plan = dev.Plan(op, precision="mixed", max_batch=B, tuning=tuning)
     X = torch.as_tensor(seeded_states(B, n3), device="cuda")
     gsteps = max(1, min(args.graph_steps, args.steps))
     graph = plan.graph(X, Ra, w["Ra_s"], nsteps=gsteps)
@@ -155,11 +215,16 @@ def main():
         "data": "synthetic",
         "config": {"workload": "N_r=40 N_fm=512 d=0.35 Pr=1 Tau=1/15 dt=5e-3 Ra_s=500 "
                                "Ra_T in [2500,3000]; batched mixed precision, HIP graph",
-                   "batch": B, "graph_steps": gsteps, "states_finite": finite},
+                   "batch": B, "graph_steps": gsteps, "states_finite": finite,
+                   "launch_shape": plan.tuning()},
         "plain_fp64_gpu": {"steps_per_s": plain, "ms_per_step": 1e3 * el64 / n64},
         "speedup_vs_plain_fp64": value / plain,
         "roofline": roof,
     }
+    if not args.no_sweep:
+        line["sweep_mixed_sizes"] = sweep_c4(table)
+    if not args.no_serving:
+        line["serving"] = serving_run(args.serve_rate, args.serve_seconds, table)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line))
