@@ -70,7 +70,8 @@ typedef struct sddc_plan sddc_plan;
  * autotuner varies them).  0 leaves a field to the library's default.
  *   chain_segment    rows per segment of the parity chains of the solves
  *                    (>= rows per chain: one segment, no join)
- *   solve_instances  instances per solve workgroup: 1, 2 or 4 */
+ *   solve_instances  instances per solve workgroup: 1, 2 or 4 (quad-lane
+ *                    kernels), 16 (MFMA kernel; best for large batches) */
 typedef struct sddc_tuning {
   int chain_segment;
   int solve_instances;

@@ -34,12 +34,15 @@ def bucket(B):
     return BUCKETS[-1]
 
 
-def candidates(N_fm):
-    """Launch shapes tried for a class with N_fm modes (N_fm/2 rows per chain)."""
+def candidates(N_fm, B=1):
+    """Launch shapes tried for a class with N_fm modes (N_fm/2 rows per chain)
+    at batch B: quad-lane solves with 1, 2 or 4 instances per workgroup, and
+    from B = 16 on the 16-instance MFMA solve."""
     n = N_fm // 2
     segs = sorted({n, max(8, 4 * math.ceil(math.sqrt(n) / 4)),
                    max(8, 4 * math.ceil(2 * math.sqrt(n) / 4))})
-    return [{"chain_segment": m, "solve_instances": nb} for m in segs for nb in (1, 2, 4)]
+    nbs = (1, 2, 4, 16) if B >= 16 else (1, 2, 4)
+    return [{"chain_segment": m, "solve_instances": nb} for m in segs for nb in nbs]
 
 
 def _key(N_fm, N_r, precision, B):
@@ -107,7 +110,7 @@ def tune_class(N_fm, N_r, B, precision="mixed", steps=10):
     """Best launch shape for one class and batch size: (tuning, {shape: µs})."""
     timings = {}
     best, best_t = None, float("inf")
-    for t in candidates(N_fm):
+    for t in candidates(N_fm, B):
         us = time_shape(N_fm, N_r, B, precision, t, steps=steps)
         timings["seg%d_nb%d" % (t["chain_segment"], t["solve_instances"])] = round(us, 2)
         if us < best_t:

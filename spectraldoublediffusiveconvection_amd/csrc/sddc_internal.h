@@ -85,6 +85,8 @@ struct Plan {
   double* chain_D2 = nullptr;                         // D2 of the A4 solve, NRP x NRS
   double* chain_Q[3] = {nullptr, nullptr, nullptr};   // row corrections (nseg > 1)
   double* chain_E[3] = {nullptr, nullptr, nullptr};   // segment exit maps (nseg > 1)
+  double* chain_Lm[3] = {nullptr, nullptr, nullptr};  // MFMA-packed block inverses
+  double* chain_D2m = nullptr;                        // MFMA-packed D2 (A4)
   TableSet tabM, tabKE;
   // workspace (device), sized for max_batch instances
   double* P = nullptr;

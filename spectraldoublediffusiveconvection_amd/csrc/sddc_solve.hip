@@ -53,6 +53,8 @@ struct ChainArgs {
   const double* D2q;        // packed D2 of the A4 solve
   const double* Qq[3];      // packed corrections, index jj
   const double* Eq[3];      // packed segment exit maps, index chain*nseg + seg
+  const double* Lm[3];      // MFMA-packed block inverses (16-instance kernel)
+  const double* D2m;        // MFMA-packed D2 of the A4 solve
   const double* vecs;
   double* EX;               // zero-entry exit states [b][3][2][nseg][2*NRP]
   double* scratch;          // 2N + 256 doubles: sink of stores of padded instances/rows
@@ -553,6 +555,192 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_join(ChainArgs a
     join_nab2<NRS, NB>(a, field, chain, start, seg, k0, k1, in, xs);
 }
 
+// ---------------------------------------------------------------------------
+// k_chain_mfma: the local (segment) chain solve for 16 instances per
+// workgroup on MFMA.  Wavefront w owns block rows 16w..16w+15 as one M tile
+// of __builtin_amdgcn_mfma_f64_16x16x4f64; the 16 instances are the N columns.
+// A fragments: lane l holds block[16w + l%16][4ks + l/16] (MFMA-packed, one
+// 16-byte load per two k steps); B fragments: the right-hand sides, exchanged
+// through LDS as [row][instance]; output: lane l, register v holds row
+// 16w + l/16 + 4v of instance l%16, which is also where the running state of
+// the chain lives.  Results equal the quad kernel's up to summation order.
+// ---------------------------------------------------------------------------
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+template <int NRS>
+__device__ __forceinline__ void mfma_block(const double2 (&a)[NRS / 8], const double* xs,
+                                           int lane, v4d_t& acc) {
+  // xs: [NRS][16] right-hand sides; B fragment of k step ks: row 4ks + lane/16
+#pragma unroll
+  for (int kp = 0; kp < NRS / 8; ++kp) {
+    const double b0 = xs[(8 * kp + (lane >> 4)) * 16 + (lane & 15)];
+    const double b1 = xs[(8 * kp + 4 + (lane >> 4)) * 16 + (lane & 15)];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kp].x, b0, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kp].y, b1, acc, 0, 0, 0);
+  }
+}
+
+template <int NRS>
+__global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a) {
+  using S = CShape<NRS>;
+  constexpr int NKP = NRS / 8;                 // 16-byte A loads per lane per block
+  constexpr int MBLK = S::NRP * NRS;           // doubles per MFMA-packed block
+  __shared__ __align__(16) double xs[2][NRS * 16 + 64];
+  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y;
+  if (!((a.fields_mask >> field) & 1)) return;
+  const int K = a.K, nr = a.nr;
+  const int start = K - 1 - chain;
+  const int n = start / 2 + 1;
+  const int k0 = seg * a.m;
+  if (k0 >= n) return;
+  const int k1 = k0 + a.m < n ? k0 + a.m : n;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int c = lane & 15;                     // instance column
+  const int b = blockIdx.z * 16 + c;
+  const bool valid = b < a.B;
+  const int bc = valid ? b : a.B - 1;
+  const size_t N = (size_t)K * nr;
+  const double* gp = a.G + (size_t)bc * a.gstride + a.goff[field];
+  double* xp = valid ? a.X + (size_t)b * a.xstride + a.xoff[field] : a.scratch;
+  double* pp = (a.P && valid) ? a.P + (size_t)b * N : a.scratch + N;
+  double* exb = a.EX + ((((size_t)(valid ? b : bc) * 3 + field) * 2 + chain) * a.nseg + seg) *
+                           2 * S::NRP;
+  double* sink = a.scratch + 2 * N + threadIdx.x;
+  int rv[4], rrv[4], wpos[4];
+  bool okr[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    rv[v] = 16 * w + (lane >> 4) + 4 * v;
+    okr[v] = rv[v] < nr;
+    rrv[v] = okr[v] ? rv[v] : nr - 1;
+    wpos[v] = rv[v] < NRS ? rv[v] * 16 + c : NRS * 16 + lane;   // tail slot for rows >= NRS
+  }
+  const bool sym_skip = a.symmetric && (field == 0 ? chain == 1 : chain == 0);
+  if (sym_skip) {   // Eq_SYM: this chain is zero (Matrix_Operators.py:1054,1166)
+    for (int k = k0; k < k1; ++k) {
+      const int j = start - 2 * k;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        *((okr[v] && valid) ? xp + (size_t)j * nr + rv[v] : sink) = 0.0;
+        if (field == 0) *((okr[v] && valid) ? pp + (size_t)j * nr + rv[v] : sink) = 0.0;
+      }
+    }
+    return;
+  }
+  const double* lb = a.Lm[field] + (size_t)w * NKP * 128 + 2 * lane;
+  double2 blk[kDepth][NKP];
+  double gv[kDepth][4];
+  auto fetch = [&](int k, double2 (&bq)[NKP], double (&g)[4]) {
+    const int j = start - 2 * (k < k1 ? k : k1 - 1);
+    const double* p = lb + (size_t)(K - 1 - j) * MBLK;
+#pragma unroll
+    for (int q = 0; q < NKP; ++q) bq[q] = *reinterpret_cast<const double2*>(p + q * 128);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) g[v] = gp[(size_t)j * nr + rrv[v]];
+  };
+#pragma unroll
+  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  if (field != 0) {
+    // NAB2 (Matrix_Operators.py:1061-1083)
+    const double dtf = a.dtf[field];
+    double bs[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int kb = k0; kb < k1; kb += kDepth) {
+#pragma unroll
+      for (int d = 0; d < kDepth; ++d) {
+        const int k = kb + d;
+        const bool live = k < k1;
+        const int j = start - 2 * (live ? k : k1 - 1);
+        const double cj = (j == 0) ? 0.5 : 1.0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xs[d & 1][wpos[v]] = gv[d][v] - cj * bs[v];
+        lds_barrier();
+        v4d_t acc = {0.0, 0.0, 0.0, 0.0};
+        mfma_block<NRS>(blk[d], xs[d & 1], lane, acc);
+        const double alpha = 2.0 * dtf * j;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          *((live && okr[v] && valid) ? xp + (size_t)j * nr + rv[v] : sink) = acc[v];
+          bs[v] = live ? bs[v] + alpha * acc[v] : bs[v];
+        }
+        fetch(k + kDepth, blk[d], gv[d]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) *((okr[v] && valid) ? exb + rv[v] : sink) = bs[v];
+    return;
+  }
+  // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt
+  const double dta = a.dtf[0];
+  double2 d2[NKP];
+  {
+    const double* p = a.D2m + (size_t)w * NKP * 128 + 2 * lane;
+#pragma unroll
+    for (int q = 0; q < NKP; ++q) d2[q] = *reinterpret_cast<const double2*>(p + q * 128);
+  }
+  double ir4[4], ir2[4], fe[4], bfe[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    ir4[v] = a.vecs[V_IR4A4 * S::NRP + rrv[v]];
+    ir2[v] = a.vecs[V_IR2A4 * S::NRP + rrv[v]];
+    fe[v] = bfe[v] = 0.0;
+  }
+  for (int kb = k0; kb < k1; kb += kDepth) {
+#pragma unroll
+    for (int d = 0; d < kDepth; ++d) {
+      const int k = kb + d;
+      const bool live = k < k1;
+      const int row = start - 2 * (live ? k : k1 - 1);
+      const double jp = (double)(row + 1);
+      const double bj = -jp * (jp + 1.0);
+      const double bjt = -2.0 * jp;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        *((live && okr[v] && valid) ? pp + (size_t)row * nr + rv[v] : sink) = fe[v];
+        xs[0][wpos[v]] = fe[v];
+      }
+      lds_barrier();
+      v4d_t l1 = {0.0, 0.0, 0.0, 0.0};
+      mfma_block<NRS>(d2, xs[0], lane, l1);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const double l1fe = l1[v] + bj * (ir4[v] * fe[v]);
+        const double bb = (dta * bjt) * (l1fe + ir4[v] * bfe[v]) - bjt * (ir2[v] * fe[v]);
+        xs[1][wpos[v]] = gv[d][v] + bb;
+      }
+      lds_barrier();
+      v4d_t acc = {0.0, 0.0, 0.0, 0.0};
+      mfma_block<NRS>(blk[d], xs[1], lane, acc);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        *((live && okr[v] && valid) ? xp + (size_t)row * nr + rv[v] : sink) = acc[v];
+        const double nbfe = bfe[v] + bj * acc[v] + bjt * fe[v];
+        bfe[v] = live ? nbfe : bfe[v];
+        fe[v] = live ? fe[v] + acc[v] : fe[v];
+      }
+      fetch(k + kDepth, blk[d], gv[d]);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    *((okr[v] && valid) ? exb + rv[v] : sink) = fe[v];
+    *((okr[v] && valid) ? exb + S::NRP + rv[v] : sink) = bfe[v];
+  }
+}
+
+template <int NRS>
+static int launch_chain_mfma_t(const ChainArgs& a, int B, hipStream_t s) {
+  using S = CShape<NRS>;
+  const dim3 grid(6, a.nseg, (B + 15) / 16), block(64 * S::WR);
+  hipLaunchKernelGGL((k_chain_mfma<NRS>), grid, block, 0, s, a);
+  if (int rc = hip_check(hipGetLastError(), "k_chain_mfma")) return rc;
+  if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup
+    const dim3 gj(6, a.nseg, (B + 1) / 2);
+    hipLaunchKernelGGL((k_chain_join<NRS, 2>), gj, block, 0, s, a);
+    if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
+  }
+  return SDDC_OK;
+}
+
 template <int NRS, int NB>
 static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
   using S = CShape<NRS>;
@@ -576,7 +764,25 @@ static int dispatch_nb(const ChainArgs& a, int NB, int B, hipStream_t s) {
   }
 }
 
+template <int NRS>
+static int dispatch_mfma_nrs(const ChainArgs& a, int B, hipStream_t s) {
+  return launch_chain_mfma_t<NRS>(a, B, s);
+}
+
 static int dispatch_chain(const ChainArgs& a, int nrs, int NB, int B, hipStream_t s) {
+  if (NB == 16) {
+    switch (nrs) {
+      case 8: return dispatch_mfma_nrs<8>(a, B, s);
+      case 16: return dispatch_mfma_nrs<16>(a, B, s);
+      case 24: return dispatch_mfma_nrs<24>(a, B, s);
+      case 32: return dispatch_mfma_nrs<32>(a, B, s);
+      case 40: return dispatch_mfma_nrs<40>(a, B, s);
+      case 48: return dispatch_mfma_nrs<48>(a, B, s);
+      case 56: return dispatch_mfma_nrs<56>(a, B, s);
+      case 64: return dispatch_mfma_nrs<64>(a, B, s);
+      default: return fail(SDDC_ERR_UNSUPPORTED, "unsupported radial size");
+    }
+  }
   switch (nrs) {
     case 8: return dispatch_nb<8>(a, NB, B, s);
     case 16: return dispatch_nb<16>(a, NB, B, s);
@@ -600,6 +806,7 @@ int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
   a.X = X;
   a.P = Pout;
   for (int f = 0; f < 3; ++f) {
+    a.Lm[f] = p.chain_Lm[f];
     a.Lq[f] = p.chain_L[f];
     a.Qq[f] = p.chain_Q[f];
     a.Eq[f] = p.chain_E[f];
@@ -607,6 +814,7 @@ int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
     a.xoff[f] = xoff[f];
   }
   a.D2q = p.chain_D2;
+  a.D2m = p.chain_D2m;
   a.vecs = p.vecs;
   a.EX = p.chain_EX;
   a.scratch = p.solve_scratch;
@@ -623,7 +831,7 @@ int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
   a.dtf[1] = p.dt;
   a.dtf[2] = p.Tau * p.dt;   // NAB2 of S uses Tau*dt (Main.py:281)
   int NB = B >= 64 ? 4 : (B >= 8 ? 2 : 1);
-  if (p.tune_solve_nbi > 0) NB = p.tune_solve_nbi;
+  if (p.tune_solve_nbi > 0) NB = p.tune_solve_nbi;   // 16: MFMA chain kernel
   return dispatch_chain(a, p.NRS, NB, B, s);
 }
 
@@ -683,6 +891,23 @@ static void pack_state(const double* A, int nr, int NRS, int NRP, double* out) {
 // segments, which avoid the corrections' extra work (measured at n = 256 on
 // MI355X).  SDDC_CHAIN_SEGMENT overrides.  Fixed per plan, so that the bits
 // of a result do not depend on the batch it was computed in.
+// MFMA packing of an nrows x ncols row-major matrix into an NRP x NC tile:
+// element (16w + l%16, 4ks + l/16) at ((w*(NC/8) + ks/2)*64 + l)*2 + ks%2, so
+// that lane l of wavefront w loads its A fragments of k steps 2kp, 2kp+1 with
+// one 16-byte load (coalesced across the wavefront).
+static void pack_mfma(const double* A, int nrows, int ncols, int lda, int NRP, int NC,
+                      double* out) {
+  const int NKP = NC / 8;
+  for (int w = 0; w < NRP / 16; ++w)
+    for (int kp = 0; kp < NKP; ++kp)
+      for (int l = 0; l < 64; ++l)
+        for (int h = 0; h < 2; ++h) {
+          const int r = 16 * w + (l & 15), c = 4 * (2 * kp + h) + (l >> 4);
+          out[((size_t)(w * NKP + kp) * 64 + l) * 2 + h] =
+              (r < nrows && c < ncols) ? A[(size_t)r * lda + c] : 0.0;
+        }
+}
+
 static int default_segment(int n, int max_batch) {
   if (const char* env = std::getenv("SDDC_CHAIN_SEGMENT")) {
     const int m = std::atoi(env);
@@ -720,6 +945,15 @@ int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment) 
   std::vector<double> d2(BLK);
   pack_quad(ops->D2_A4, nr, nr, nr, NRP, NRS, d2.data());
   if (int rc = upload_vec(&p.chain_D2, d2)) return rc;
+  if (p.tune_solve_nbi == 16) {   // operands of the MFMA chain kernel
+    for (int f = 0; f < 3; ++f) {
+      for (int jj = 0; jj < K; ++jj)
+        pack_mfma(inv[f] + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
+      if (int rc = upload_vec(&p.chain_Lm[f], buf)) return rc;
+    }
+    pack_mfma(ops->D2_A4, nr, nr, nr, NRP, NRS, d2.data());
+    if (int rc = upload_vec(&p.chain_D2m, d2)) return rc;
+  }
   if (p.nseg == 1) return SDDC_OK;   // one segment per chain: no joins
 
   // NAB2 (fields 1, 2).  State b; T maps the entry state to b at the row:
@@ -807,6 +1041,12 @@ void free_chain(Plan& p) {
   }
   if (p.chain_D2) (void)hipFree(p.chain_D2);
   p.chain_D2 = nullptr;
+  for (int f = 0; f < 3; ++f) {
+    if (p.chain_Lm[f]) (void)hipFree(p.chain_Lm[f]);
+    p.chain_Lm[f] = nullptr;
+  }
+  if (p.chain_D2m) (void)hipFree(p.chain_D2m);
+  p.chain_D2m = nullptr;
 }
 
 }  // namespace sddc

@@ -15,6 +15,7 @@ def test_autotune_buckets_and_candidates(tmp_path):
     c = autotune.candidates(512)
     segs = sorted({t["chain_segment"] for t in c})
     assert segs == [16, 32, 256] and {t["solve_instances"] for t in c} == {1, 2, 4}
+    assert {t["solve_instances"] for t in autotune.candidates(512, 64)} == {1, 2, 4, 16}
     t = autotune.TuningTable(str(tmp_path / "t.json"))
     t.put(64, 20, "mixed", 10, {"chain_segment": 8, "solve_instances": 2}, {"x": 1.0})
     t.save()
@@ -71,7 +72,7 @@ def test_tuned_plans_agree_with_oracle():
     o = O.build_operators(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0)
     X = 1e-3 * np.random.default_rng(8).standard_normal((3, 3 * K * (Nr - 1)))
     ref = [O.step(x, o, 3000.0, 500.0) for x in X]
-    for t in autotune.candidates(K):
+    for t in autotune.candidates(K, 16):
         p = dev.Plan(ops.build(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0), max_batch=3, tuning=t)
         y = p.step(torch.as_tensor(X, device="cuda"), 3000.0, 500.0).cpu().numpy()
         for a, b in zip(y, ref):

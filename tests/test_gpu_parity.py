@@ -148,6 +148,42 @@ def test_solve_segmentations(K, Nr, max_batch, sym):
         assert rel(Y[b], ref) < 1e-12, b
 
 
+@pytest.mark.parametrize("K,Nr", [(64, 20), (512, 40), (10, 9), (32, 65)])
+@pytest.mark.parametrize("segment", [0, 8])
+@pytest.mark.parametrize("sym", [False, True])
+def test_solve_mfma(K, Nr, segment, sym):
+    """The 16-instance MFMA chain solve (solve_instances = 16) against the
+    oracle's back substitution, for a ragged batch of 21 (one full and one
+    partial group of 16), unsegmented and in segments of 8 rows, then three
+    steps against the oracle."""
+    d, dt, Pr, Tau = 0.35, 5e-3, 1.0, 1 / 15.0
+    o = O.build_operators(K, Nr, d, dt, Pr, Tau)
+    B = 21
+    p = dev.Plan(ops.build(K, Nr, d, dt, Pr, Tau), symmetric=sym, max_batch=B,
+                 tuning={"chain_segment": segment, "solve_instances": 16})
+    assert p.tuning()["solve_instances"] == 16
+    nr = Nr - 1
+    rng = np.random.default_rng(K + segment)
+    g = rng.standard_normal((B, K * nr)) * np.logspace(0, -3, K).repeat(nr)[None, :]
+    dts = (Pr * dt, dt, Tau * dt)
+    for field in range(3):
+        f = H(p.bsub(T(g), field))
+        for b in range(B):
+            if field == 0:
+                ref = O.a4_bsub(g[b], o.Linv_A4, o.D2_A4, o.IR4_A4, o.IR2_A4, K, nr, dts[0], sym)
+            else:
+                Linv = o.Linv_T if field == 1 else o.Linv_S
+                ref = O.nab2_bsub(g[b], Linv, K, nr, dts[field], sym)
+            assert rel(f[b], ref) < 1e-14, (field, b)
+    X = 1e-3 * rng.standard_normal((B, 3 * K * nr))
+    Y = H(p.step(T(X), RA_LONG, 500.0, nsteps=3))
+    for b in (0, 7, 15, 16, 20):
+        ref = X[b]
+        for _ in range(3):
+            ref = O.step(ref, o, RA_LONG, 500.0, symmetric=sym)
+        assert rel(Y[b], ref) < 1e-12, b
+
+
 def test_single_step_and_linearised(c1):
     o, p = c1
     for name in ("X_big", "X_small"):
