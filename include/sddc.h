@@ -190,6 +190,36 @@ int sddc_gmres(sddc_krylov* k, const double* Ra, const double* Ra_s, const doubl
 int sddc_newton(sddc_krylov* k, const double* Ra, const double* Ra_s, double* X,
                 const sddc_krylov_opts* opts, sddc_newton_info* info, void* stream);
 
+/* One pseudo-arclength continuation step in Ra_T (_ContinC, Main.py:754-950)
+ * on the device.  Y = (X, Ra_T) and Y_dot are device vectors of 3N+1 values,
+ * updated in place: predictor (tangent from DF_X xi = -DF_Ra when dt < 10 or
+ * Y_dot = 0, else Y_dot), Newton corrector on the bordered system with
+ * delta = 1/(3N) (GMRES on the device), halving ds after 5 corrector
+ * iterations, doubling it after convergence within 4, and the new unit
+ * tangent.  *ds is updated.  Ra_s is a device scalar.  The Krylov options
+ * give tol_newton, tol_gmres, rtol, max_restarts and finite_difference.
+ * info->status reports the reference's failure cases (Main.py:859,891,923)
+ * without an error code; a tangent solve that does not converge leaves
+ * Y_dot = 0 and tangent_ok = 0 (Main.py:943-944). */
+enum {
+  SDDC_ARC_OK = 0,
+  SDDC_ARC_PREDICTOR_FAILED = 1,  /* GMRES for xi did not converge (Main.py:859) */
+  SDDC_ARC_STEP_TOO_SMALL = 2,    /* ds < tol_newton after halving (Main.py:891) */
+  SDDC_ARC_CORRECTOR_FAILED = 3   /* GMRES for dY did not converge (Main.py:923) */
+};
+typedef struct sddc_arclength_info {
+  int status;             /* SDDC_ARC_*                                    */
+  int iterations;         /* corrector iterations of the accepted step     */
+  int halvings;           /* ds halvings                                   */
+  int krylov_iterations;  /* GMRES iterations, all solves                  */
+  int tangent_ok;         /* 1: the tangent solve converged                */
+  int reserved;
+  double error_X, error_mu;   /* last corrector errors (Main.py:925-926)   */
+} sddc_arclength_info;
+int sddc_arclength_step(sddc_krylov* k, const double* Ra_s, double* Y, double* Y_dot,
+                        double sign, double* ds, const sddc_krylov_opts* opts,
+                        sddc_arclength_info* info, void* stream);
+
 /* Launch one stage of the step on the plan's workspace (for timing and
  * profiling; no counterpart in the reference).  stage: 0 suffix sums,
  * 1 spectral stage, 2 nonlinear products, 3 right-hand side, 4 solves.

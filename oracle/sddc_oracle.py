@@ -25,7 +25,10 @@ How each part is pinned (tests/test_oracle.py):
     written from the reference by tests/golden/make_golden.py, plus
     bilinearity / directional-derivative / symmetry invariants;
   * Newton: residual at convergence (scipy ``lgmres`` is third-party; pinned
-    at solution level only).
+    at solution level only);
+  * pseudo-arclength step: at solution level, F(X, Ra) = 0, the arclength
+    condition, the tangent's normalisation and DG Y_dot = e_last; pdfmu
+    against a difference quotient of the (affine in Ra) step.
 """
 from __future__ import annotations
 
@@ -38,7 +41,7 @@ __all__ = [
     "eq_sym_mask", "j_theta", "dt0_theta", "a2_sine", "a2_sine_r2",
     "spectral_slabs", "nlin_fx", "nlin_dfx", "nab2_bsub", "a4_bsub",
     "step", "step_linearized", "kinetic_energy", "nusselt", "time_step",
-    "newton",
+    "newton", "pdfmu", "contin_step",
 ]
 
 
@@ -600,3 +603,106 @@ def newton(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric=False, dt=1,
     NuT = nusselt(X[N:2 * N], d, o.R, o.D, K, nr)
     NuS = nusselt(X[2 * N:3 * N], d, o.R, o.D, K, nr)
     return X, np.linalg.norm(X, 2), KE, NuT, NuS, True
+
+
+# --------------------------------------------------------------------------
+# pseudo-arclength continuation (Main.py:754-950)
+# --------------------------------------------------------------------------
+
+def pdfmu(X, o, symmetric=False):
+    """d Step / d Ra_T at X: [A4_BSub(dt Pr gr_k T), 0, 0] (PDFmu, Main.py:828-836)."""
+    K, nr = o.N_fm, o.nr
+    N = K * nr
+    X = np.asarray(X, dtype=np.float64)
+    if symmetric:
+        X = X * eq_sym_mask(K, nr)
+    T = X[N:2 * N].reshape(K, nr)
+    grv = np.zeros((K, nr))
+    grv[:K - 1] = o.grk[:K - 1] * T[1:]
+    out = np.zeros(3 * N)
+    out[:N] = a4_bsub(o.dt * o.Pr * grv.reshape(-1), o.Linv_A4, o.D2_A4, o.IR4_A4, o.IR2_A4,
+                      K, nr, o.Pr * o.dt, symmetric)
+    return out
+
+
+def contin_step(Y_0_dot, Y_0, sign, ds, o, Ra_s, symmetric=False, tol_newton=1e-08,
+                tol_gmres=1e-04, Krylov_Space_Size=300, rtol=1e-5):
+    """One pseudo-arclength step of _ContinC (Main.py:754-950) with scipy's
+    lgmres: predictor along the tangent, Newton corrector on the bordered
+    system (delta = 1/(3N)), ds halving after 5 corrector iterations and
+    doubling after convergence within 4, then the normalised tangent.
+    Returns (Y_dot, Y, sign, ds); raises ValueError where the reference does.
+    rtol is lgmres's relative tolerance (its default in the reference)."""
+    import scipy.sparse.linalg as spla
+    K, nr = o.N_fm, o.nr
+    N = K * nr
+    n = 3 * N
+    delta = 1.0 / (3.0 * N)
+    mask = eq_sym_mask(K, nr) if symmetric else 1.0
+    Y_0 = np.asarray(Y_0, dtype=np.float64)
+    Y_0_dot = np.asarray(Y_0_dot, dtype=np.float64)
+
+    def solve(A, b, atol):
+        Aop = spla.LinearOperator((b.shape[0], b.shape[0]), matvec=A, dtype="float64")
+        return spla.lgmres(Aop, b, maxiter=250, inner_m=Krylov_Space_Size, atol=atol, rtol=rtol)
+
+    def pfx(X, mu):
+        return step(X, o, mu, Ra_s, symmetric=symmetric) - X
+
+    def dfx(v, X, mu):
+        return step_linearized(v, X, o, mu, Ra_s, symmetric) - v
+
+    # predictor
+    X0 = mask * Y_0[:-1]
+    mu0 = Y_0[-1]
+    if np.linalg.norm(Y_0_dot, 2) == 0 or o.dt < 10:
+        rhs = -1.0 * pdfmu(X0, o, symmetric)
+        xi, code = solve(lambda v: dfx(v, X0, mu0), rhs, tol_newton * np.linalg.norm(rhs, 2))
+        if code != 0:
+            raise ValueError("\n LGMRES couldn't converge when predicting ξ \n")
+        mu_dot = sign / np.sqrt(1.0 + delta * (np.linalg.norm(xi, 2) - 1.0))
+        X_dot = mu_dot * xi
+    else:
+        X_dot, mu_dot = Y_0_dot[:-1].copy(), Y_0_dot[-1]
+    Y = np.concatenate((X0 + X_dot * ds, [mu0 + mu_dot * ds]))
+
+    # corrector
+    err_X, err_mu, it = 1.0, 1.0, 0
+    G = np.zeros(n + 1)
+    while err_X > tol_newton or err_mu > tol_newton or it < 2:
+        if it >= 5:
+            it = 0
+            ds *= 0.5
+            if ds < tol_newton:
+                raise ValueError("\n Step-size ds has become too small terminating \n")
+            Y = np.concatenate((X0 + X_dot * ds, [mu0 + mu_dot * ds]))
+        X = mask * Y[:-1]
+        mu = Y[-1]
+        dmu_col = pdfmu(X, o, symmetric)
+        G[:-1] = pfx(X, mu)
+        G[-1] = delta * np.dot(X_dot, X - X0) + (1.0 - delta) * mu_dot * (mu - mu0) - ds
+
+        def DG(dY, X=X, mu=mu, dmu_col=dmu_col):
+            top = dfx(dY[:-1], X, mu) + dmu_col * dY[-1]
+            return np.concatenate((top, [delta * np.dot(X_dot, dY[:-1])
+                                          + (1.0 - delta) * mu_dot * dY[-1]]))
+
+        b_norm = np.sqrt(delta * np.dot(G[:-1], G[:-1]) + (1.0 - delta) * G[-1] ** 2)
+        dY, code = solve(DG, G, tol_gmres * b_norm)
+        Y = Y - dY
+        if code != 0:
+            raise ValueError("\n LGMRES couldn't converge when predicting dY \n")
+        err_X = np.linalg.norm(dY[:-1], 2) / np.linalg.norm(X, 2)
+        err_mu = abs(dY[-1]) / abs(mu)
+        it += 1
+    if it <= 4:
+        ds *= 2
+
+    # tangent with the last corrector's operator
+    E = np.zeros(n + 1)
+    E[-1] = 1.0
+    Y_dot, code = solve(DG, E, tol_newton * 1.0)
+    if code != 0:
+        return 0.0 * Y_dot, Y, sign, ds
+    Y_dot = Y_dot / np.sqrt(delta * np.dot(Y_dot[:-1], Y_dot[:-1]) + (1.0 - delta) * Y_dot[-1] ** 2)
+    return Y_dot, Y, sign, ds

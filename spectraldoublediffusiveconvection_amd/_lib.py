@@ -59,6 +59,16 @@ class SddcNewtonInfo(ctypes.Structure):
                 ("krylov_iterations", ctypes.c_int), ("error", ctypes.c_double)]
 
 
+class SddcArclengthInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("halvings", ctypes.c_int), ("krylov_iterations", ctypes.c_int),
+                ("tangent_ok", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("error_X", ctypes.c_double), ("error_mu", ctypes.c_double)]
+
+
+ARC_OK, ARC_PREDICTOR_FAILED, ARC_STEP_TOO_SMALL, ARC_CORRECTOR_FAILED = 0, 1, 2, 3
+
+
 class SddcError(RuntimeError):
     """A libsddc call failed (HIP error or unsupported size)."""
 
@@ -98,6 +108,9 @@ _SIGS = {
                         ctypes.POINTER(SddcKrylovInfo), _V]),
     "sddc_newton": (_I, [_V, _V, _V, _V, ctypes.POINTER(SddcKrylovOpts),
                          ctypes.POINTER(SddcNewtonInfo), _V]),
+    "sddc_arclength_step": (_I, [_V, _V, _V, _V, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(SddcKrylovOpts),
+                                 ctypes.POINTER(SddcArclengthInfo), _V]),
     "sddc_last_error": (ctypes.c_char_p, []),
     "sddc_version": (ctypes.c_char_p, []),
 }

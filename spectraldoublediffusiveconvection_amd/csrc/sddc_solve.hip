@@ -95,6 +95,17 @@ __device__ __forceinline__ double quad_sum(double v) {
 // complete (lgkmcnt(0)) before s_barrier.  No vmcnt wait, so loads in flight
 // stay in flight.  Signal fences keep the compiler from moving LDS accesses
 // across it.
+// Waits for every load in flight.  Issued once after a loop's prologue
+// (the first kDepth fetches and the loop-invariant operands): without it the
+// compiler's wait-count analysis merges the prologue's issue order into the
+// loop header and waits, on every trip, for loads issued one row earlier,
+// which serialises the chain on memory latency.
+__device__ __forceinline__ void drain_loads() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);   // vmcnt 0, expcnt 0, lgkmcnt 0
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 __device__ __forceinline__ void lds_barrier() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0xC07F);   // vmcnt 63 (none), expcnt 7 (none), lgkmcnt 0
@@ -189,6 +200,7 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
   };
 #pragma unroll
   for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  drain_loads();
   double b[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) b[t] = 0.0;
@@ -254,6 +266,7 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
   };
 #pragma unroll
   for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  drain_loads();
   double fe[NB], bfe[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) fe[t] = bfe[t] = 0.0;
@@ -370,6 +383,7 @@ __device__ __forceinline__ void join_nab2(const ChainArgs& a, int field, int cha
   };
 #pragma unroll
   for (int d = 0; d < kDepthE; ++d) fetch_e(d, e[d], exv[d]);
+  drain_loads();
   double sv[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) sv[t] = 0.0;
@@ -412,6 +426,7 @@ __device__ __forceinline__ void join_nab2(const ChainArgs& a, int field, int cha
   };
 #pragma unroll
   for (int d = 0; d < kDepth; ++d) fetch_q(k0 + d, qv[d], fl[d]);
+  drain_loads();
   for (int kb = k0; kb < k1; kb += kDepth) {
 #pragma unroll
     for (int d = 0; d < kDepth; ++d) {
@@ -464,6 +479,7 @@ __device__ __forceinline__ void join_a4(const ChainArgs& a, int chain, int start
   };
 #pragma unroll
   for (int d = 0; d < kDepthE; ++d) fetch_e(d, ef[d], ebv[d], exf[d], exb[d]);
+  drain_loads();
   double sf[NB], sb[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) sf[t] = sb[t] = 0.0;
@@ -514,6 +530,7 @@ __device__ __forceinline__ void join_a4(const ChainArgs& a, int chain, int start
   };
 #pragma unroll
   for (int d = 0; d < kDepth; ++d) fetch_q(k0 + d, qv[d], fl[d]);
+  drain_loads();
   double fe[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) fe[t] = sf[t];
@@ -642,6 +659,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
   for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
   if (field != 0) {
     // NAB2 (Matrix_Operators.py:1061-1083)
+    drain_loads();
     const double dtf = a.dtf[field];
     double bs[4] = {0.0, 0.0, 0.0, 0.0};
     for (int kb = k0; kb < k1; kb += kDepth) {
@@ -684,6 +702,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
     ir2[v] = a.vecs[V_IR2A4 * S::NRP + rrv[v]];
     fe[v] = bfe[v] = 0.0;
   }
+  drain_loads();
   for (int kb = k0; kb < k1; kb += kDepth) {
 #pragma unroll
     for (int d = 0; d < kDepth; ++d) {

@@ -7,9 +7,11 @@ _Newton runs the device Newton-Krylov solver (restarted GMRES in float64 with
 CGS2 Gram-Schmidt, sddc_newton) with the reference's tolerances and limits;
 set KRYLOV = "lgmres" to use scipy's lgmres on the host around the device's
 step and linearised step instead (the reference's own Krylov method).  Both
-converge to the same steady state within the Newton tolerance.  Checkpoints
-use h5py with the reference's groups when h5py is importable, otherwise a
-NumPy .npz beside the requested name.
+converge to the same steady state within the Newton tolerance.  The
+continuation drivers (_NewtonC, _ContinC, _Continuation) run their Newton and
+pseudo-arclength steps on the device (sddc_newton, sddc_arclength_step).
+Checkpoints use h5py with the reference's groups when h5py is importable,
+otherwise a NumPy .npz beside the requested name.
 """
 from __future__ import annotations
 
@@ -164,6 +166,18 @@ KRYLOV = "device"
 _krylov = {}
 
 
+def _krylov_for(plan, restart):
+    """One Krylov basis, rebuilt when the plan (an object the registry may
+    have closed and replaced) or the restart length changes."""
+    from ..newton import Krylov
+    kr = _krylov.get("k")
+    if kr is None or kr.plan is not plan or kr.restart != int(restart):
+        if kr is not None:
+            kr.close()
+        kr = _krylov["k"] = Krylov(plan, restart=restart)
+    return kr
+
+
 def _Newton(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt=1, tol_newton=1e-08,
             tol_gmres=1e-04, Krylov_Space_Size=300):
     """Steady state by Newton iteration on Step(X) - X (Main.py:430-554).
@@ -171,15 +185,8 @@ def _Newton(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt=1, tol_newton=1e-0
     if KRYLOV == "lgmres":
         return _newton_lgmres(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt, tol_newton,
                               tol_gmres, Krylov_Space_Size)
-    from ..newton import Krylov
     plan = _reg.step_plan(N_fm, N_r, d, dt, Pr, Tau, symmetric)
-    # one Krylov basis, rebuilt when the plan (an object the registry may have
-    # closed and replaced) or the restart length changes
-    kr = _krylov.get("k")
-    if kr is None or kr.plan is not plan or kr.restart != int(Krylov_Space_Size):
-        if kr is not None:
-            kr.close()
-        kr = _krylov["k"] = Krylov(plan, restart=Krylov_Space_Size)
+    kr = _krylov_for(plan, Krylov_Space_Size)
     Xd, info = kr.newton(_dev(X), Ra, Ra_s, tol_newton=tol_newton, tol_gmres=tol_gmres,
                          max_newton=5, max_restarts=250)
     print('Newton Iterations = %d, Error = %e, Krylov iterations = %d'
@@ -235,3 +242,157 @@ def _newton_lgmres(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt, tol_newton
     NuT = Nusselt(X[N:2 * N], d, R, D, N_fm, nr)
     NuS = Nusselt(X[2 * N:3 * N], d, R, D, N_fm, nr)
     return X, Norm, KE, NuT, NuS, True
+
+
+# --------------------------------------------------------------------------
+# continuation in Ra_T (Main.py:677-1045)
+# --------------------------------------------------------------------------
+
+class result():
+    """Record of a continuation run (Main.py:677-713)."""
+
+    def __init__(self):
+        self.NuT = []
+        self.NuS = []
+        self.KE = []
+        self.Norm = []
+        self.Ra = []
+        self.Ra_dot = []
+        self.Y_FOLD = []
+        self.X_DATA = []
+        self.Ra_DATA = []
+        self.Iterations = 0
+
+    def __str__(self):
+        i = self.Iterations
+        return ('Continuation succeed \n'
+                + 'Total iterations  = ' + str(i) + '\n'
+                + 'Ra                = ' + str(self.Ra[i]) + '\n'
+                + 'Ra_dot            = ' + str(self.Ra_dot[i]) + '\n'
+                + 'NuT               = ' + str(self.NuT[i]) + '\n'
+                + 'NuS               = ' + str(self.NuS[i]) + '\n'
+                + 'KE                = ' + str(self.KE[i]) + '\n')
+
+
+def _NewtonC(Y, sign, ds, **kwargs_f):
+    """Natural-parameter step: Newton at Ra_T + sign*ds from Y = (X, Ra_T);
+    ds doubles on success and halves on failure (Main.py:717-739)."""
+    X = Y[0:-1].copy()
+    Ra = Y[-1].copy() + sign * ds
+    kwargs_f["Ra"] = Ra
+    X_new, Norm, KE, NuT, NuS, exitcode = _Newton(X, **kwargs_f, tol_newton=1e-08)
+    if exitcode is True:
+        ds *= 2
+        return np.hstack((X_new, Ra)), sign, ds, Norm, KE, NuT, NuS, exitcode
+    ds *= 0.5
+    return Y, sign, ds, Norm, KE, NuT, NuS, exitcode
+
+
+FINITE_DIFFERENCE = False   # J v of the continuation by finite differences of the step
+
+
+def _ContinC(Y_0_dot, Y_0, sign, ds, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, dt=1,
+             tol_newton=1e-08, tol_gmres=1e-04, Krylov_Space_Size=300):
+    """One pseudo-arclength step from Y_0 = (X, Ra_T) (Main.py:742-950):
+    returns (Y_dot, Y, sign, ds, Norm, KE, NuT, NuS, True) or raises
+    ValueError as the reference does.  The predictor, the corrector on the
+    bordered system and the tangent run on the device (sddc_arclength_step)."""
+    from .._lib import ARC_CORRECTOR_FAILED, ARC_PREDICTOR_FAILED, ARC_STEP_TOO_SMALL
+    plan = _reg.step_plan(N_fm, N_r, d, dt, Pr, Tau, symmetric)
+    kr = _krylov_for(plan, Krylov_Space_Size)
+    Yd, Ydd, ds, info = kr.arclength_step(_dev(Y_0), _dev(Y_0_dot), Ra_s, sign, ds,
+                                          tol_newton=tol_newton, tol_gmres=tol_gmres,
+                                          fd=FINITE_DIFFERENCE)
+    for _ in range(info["halvings"]):
+        print('Reducing the step-size ds by half \n')
+    if info["status"] == ARC_PREDICTOR_FAILED:
+        raise ValueError("\n LGMRES couldn't converge when predicting ξ \n")
+    if info["status"] == ARC_STEP_TOO_SMALL:
+        raise ValueError("\n Step-size ds has become too small terminating \n")
+    if info["status"] == ARC_CORRECTOR_FAILED:
+        raise ValueError("\n LGMRES couldn't converge when predicting dY \n")
+    print('Psuedo-Arc Iterations = %d, Error X = %e, Error µ = %e'
+          % (info["iterations"], info["error_X"], info["error_mu"]), "\n")
+    ops = plan.ops
+    R, D = ops.R, ops.D
+    nr = N_r - 1
+    N = N_fm * nr
+    Y = _host(Yd.view(1, -1))
+    Y_dot = _host(Ydd.view(1, -1))
+    X = Y[0:-1].copy()
+    Norm = np.linalg.norm(X, 2)
+    KE = Kinetic_Energy(X, R, D, N_fm, nr, symmetric)
+    NuT = Nusselt(X[N:2 * N], d, R, D, N_fm, nr)
+    NuS = Nusselt(X[2 * N:3 * N], d, R, D, N_fm, nr)
+    return Y_dot, Y, sign, ds, Norm, KE, NuT, NuS, True
+
+
+def _write_continuation(filename, Result, params):
+    """Checkpoints, Parameters and Bifurcation groups (Main.py:1024-1040)."""
+    try:
+        import h5py
+    except ImportError:
+        h5py = None
+    if h5py is not None:
+        with h5py.File(filename, "w") as f:
+            g = f.create_group("Checkpoints")
+            g["X_DATA"] = Result.X_DATA
+            g["Ra_DATA"] = Result.Ra_DATA
+            g = f.create_group("Parameters")
+            for k, v in params.items():
+                g[k] = v
+            g = f.create_group("Bifurcation")
+            for k, v in vars(Result).items():
+                g.create_dataset(k, data=v)
+        return filename
+    data = {"Checkpoints/X_DATA": np.asarray(Result.X_DATA),
+            "Checkpoints/Ra_DATA": np.asarray(Result.Ra_DATA)}
+    data.update({"Parameters/" + k: np.asarray(v) for k, v in params.items()})
+    data.update({"Bifurcation/" + k: np.asarray(v) for k, v in vars(Result).items()})
+    np.savez(filename + ".npz", **data)
+    return filename + ".npz"
+
+
+def _Continuation(filename, N_steps, sign, Y, **kwargs):
+    """Branch continuation in Ra_T from Y = (X, Ra_T) for N_steps points
+    (Main.py:958-1045): natural-parameter Newton steps while ds > 1, switching
+    to pseudo-arclength below, fold detection by a sign change of Ra_dot, and a
+    checkpoint every fifth point.  Returns the result record (the reference
+    returns None after writing the same record)."""
+    Y = Y.copy()
+    Y_dot = 0. * Y
+    ds, ds_min, ds_max = 0.01, 1.0, 10.0
+    Result = result()
+    Y_dot_new = None   # the last pseudo-arclength tangent, kept across Newton steps
+    while Result.Iterations < N_steps:
+        if ds > ds_min:
+            OUT = _NewtonC(Y, sign, ds, **kwargs)
+            if OUT[-1] is False:
+                print('Switching to arc-length \n')
+                Y_dot_new, Y_new, sign, ds, Norm, KE, NuT, NuS, exitcode = _ContinC(
+                    Y_dot, Y, sign, ds, **kwargs)
+            else:
+                Y_new, sign, ds, Norm, KE, NuT, NuS = OUT[:7]
+            if ds > ds_max:
+                ds = ds_max
+        else:
+            Y_dot_new, Y_new, sign, ds, Norm, KE, NuT, NuS, exitcode = _ContinC(
+                Y_dot, Y, sign, ds, **kwargs)
+            if Y_dot_new[-1] * Y_dot[-1] < 0:   # fold (Main.py:995-997)
+                Result.Y_FOLD.append(Y_new)
+            sign = 1.0 if Y_new[-1] > Y[-1] else -1.0
+        Y = Y_new.copy()
+        Y_dot = Y_dot_new.copy() if Y_dot_new is not None else 0. * Y
+        Result.Ra.append(Y[-1])
+        Result.Ra_dot.append(Y_dot[-1])
+        Result.Norm.append(Norm)
+        Result.KE.append(KE)
+        Result.NuT.append(NuT)
+        Result.NuS.append(NuS)
+        if Result.Iterations % 5 == 0:
+            Result.X_DATA.append(Y[0:-1])
+            Result.Ra_DATA.append(Y[-1])
+            _write_continuation(filename, Result, kwargs)
+        print(Result, flush=True)
+        Result.Iterations += 1
+    return Result

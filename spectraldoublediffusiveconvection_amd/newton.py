@@ -1,10 +1,12 @@
-"""Device Newton-Krylov steady solves (sddc_krylov_*, sddc_gmres, sddc_newton).
+"""Device Newton-Krylov steady solves and pseudo-arclength continuation
+(sddc_krylov_*, sddc_gmres, sddc_newton, sddc_arclength_step).
 
 Krylov(plan, restart) owns the Krylov basis of one plan.  ``newton`` runs the
 Newton iteration of _Newton (Main.py:430-554) with restarted GMRES on the
 device (float64, CGS2 Gram-Schmidt); ``gmres`` solves one linearised system
-J x = b with J = PDFX (Main.py:498-521).  States are (3N,) or (1, 3N)
-float64 CUDA tensors.
+J x = b with J = PDFX (Main.py:498-521); ``arclength_step`` is one step of
+_ContinC (Main.py:754-950) on Y = (X, Ra_T).  States are (3N,) or (1, 3N)
+float64 CUDA tensors; Y and its tangent are (3N+1,).
 """
 from __future__ import annotations
 
@@ -79,3 +81,29 @@ class Krylov:
                                           ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
         return X, {"converged": bool(info.converged), "iterations": info.iterations,
                    "krylov_iterations": info.krylov_iterations, "error": info.error}
+
+    def arclength_step(self, Y, Y_dot, Ra_s, sign, ds, tol_newton=1e-8, tol_gmres=1e-4,
+                       rtol=1e-5, max_restarts=250, fd=False):
+        """One pseudo-arclength step from Y = (X, Ra_T) with tangent Y_dot
+        (zeros: computed).  Returns (Y_new, Y_dot_new, ds_new, info dict);
+        info["status"] is one of _lib.ARC_* (the reference raises ValueError
+        for the failures)."""
+        n = 3 * self.plan.N
+        Y = Y.reshape(-1).contiguous().clone()
+        Y_dot = Y_dot.reshape(-1).contiguous().clone()
+        for v in (Y, Y_dot):
+            if v.dtype != torch.float64 or not v.is_cuda or v.numel() != n + 1:
+                raise ValueError("Y and Y_dot must be float64 CUDA tensors of length 3N+1")
+        ras = self.plan._params(Ra_s, 1)
+        d = ctypes.c_double(float(ds))
+        info = _lib.SddcArclengthInfo()
+        o = self._opts(tol_newton=tol_newton, tol_gmres=tol_gmres, rtol=rtol,
+                       max_restarts=max_restarts, fd=fd)
+        _lib.check(_lib.lib().sddc_arclength_step(
+            self._h, _ptr(ras), _ptr(Y), _ptr(Y_dot), float(sign), ctypes.byref(d),
+            ctypes.byref(o), ctypes.byref(info),
+            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return Y, Y_dot, d.value, {
+            "status": info.status, "iterations": info.iterations, "halvings": info.halvings,
+            "krylov_iterations": info.krylov_iterations, "tangent_ok": bool(info.tangent_ok),
+            "error_X": info.error_X, "error_mu": info.error_mu}
