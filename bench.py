@@ -95,8 +95,8 @@ def sweep_c4(tuning_table, n=256, graph_steps=10, launches=5):
 def serving_run(rate, seconds, tuning_table):
     """Poisson stream of mixed jobs through the serving engine: 60% time
     integrations (500 steps, sizes as in the sweep), 30% Newton solves and 10%
-    20-point continuations in Ra_T of steady wide-gap convection (d=2, Pr=10,
-    N_r=16, N_fm=32, started near the steady state)."""
+    20-point pseudo-arclength continuations in Ra_T of steady wide-gap
+    convection (d=2, Pr=10, N_r=16, N_fm=32, started near the steady state)."""
     import torch
 
     from spectraldoublediffusiveconvection_amd import device as dev
@@ -110,14 +110,15 @@ def serving_run(rate, seconds, tuning_table):
                 symmetric=True, restart=100, dRa=10.0)
     w = WORKLOAD
     jobs = serving.poisson_stream(rate, seconds, seed=7, newton_case=case,
-                                  ra=(w["Ra_lo"], w["Ra_hi"]))
+                                  ra=(w["Ra_lo"], w["Ra_hi"]), arclength=True)
     eng = serving.Engine(precision="mixed", capacity=64, chunk=25, newton_workers=3,
                          tuning_table=tuning_table)
     eng.run(jobs, realtime=True, max_seconds=seconds * 10 + 30)
     eng.shutdown()
     st = eng.stats()
     st.update({"rate_jobs_per_s": rate, "arrival_window_s": seconds, "submitted": len(jobs),
-               "mix": "60% time (500 steps), 30% newton, 10% continuation (20 points)"})
+               "mix": "60% time (500 steps), 30% newton, "
+                      "10% pseudo-arclength continuation (20 points)"})
     return st
 
 

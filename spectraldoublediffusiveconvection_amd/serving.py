@@ -11,11 +11,14 @@ first job is admitted, if its HBM footprint fits the budget; a class with no
 work is released when the budget is needed for another.  Each class steps on
 its own stream, so classes overlap on the device.
 
-Newton jobs (steady states, device Newton-Krylov) and continuation jobs (a
-sequence of Newton solves along a path of Ra_T values, each started from the
-previous solution) run on worker threads with their own streams; their GPU
-work overlaps with the batched steps (ctypes releases the GIL during library
-calls).
+Newton jobs (steady states, device Newton-Krylov) and continuation jobs run
+on worker threads with their own streams; their GPU work overlaps with the
+batched steps (ctypes releases the GIL during library calls).  A
+continuation job converges the steady state at its Ra_T and then either
+follows a given path of Ra_T values by Newton solves, each started from the
+previous solution (natural parameter, _NewtonC, Main.py:717-739), or takes
+`arc_points` pseudo-arclength steps (_ContinC, Main.py:742-950) with step
+size control, which also passes folds of the branch.
 
 The engine records arrival, admission and completion times of every job;
 ``Engine.stats()`` gives p50/p99 latency (completion - arrival) and
@@ -51,7 +54,10 @@ class Job:
     Tau: float = 1.0 / 15.0
     symmetric: bool = False
     nsteps: int = 500              # time integration
-    ra_path: Optional[List[float]] = None   # continuation: Ra_T values after Ra
+    ra_path: Optional[List[float]] = None   # natural continuation: Ra_T values after Ra
+    arc_points: int = 0            # pseudo-arclength continuation: points after Ra
+    ds: float = 10.0               # its initial arclength step
+    sign: float = 1.0              # its initial direction in Ra_T
     tol_newton: float = 1e-8
     tol_gmres: float = 1e-4
     restart: int = 100
@@ -188,7 +194,21 @@ class NewtonWorker(threading.Thread):
             infos.append(info)
             points.append((ra, X.cpu().numpy()))
             if not info["converged"]:
-                break
+                return points, infos
+        if job.arc_points > 0:
+            Y = torch.cat((X.reshape(-1), torch.tensor([float(job.Ra)], dtype=torch.float64,
+                                                        device=X.device)))
+            Y_dot = torch.zeros_like(Y)
+            ds = job.ds
+            for _ in range(job.arc_points):
+                Y, Y_dot, ds, info = kr.arclength_step(Y, Y_dot, job.Ra_s, job.sign, ds,
+                                                       tol_newton=job.tol_newton,
+                                                       tol_gmres=job.tol_gmres)
+                infos.append(info)
+                if info["status"] != 0:
+                    break
+                y = Y.cpu().numpy()
+                points.append((float(y[-1]), y[:-1]))
         return points, infos
 
     def run(self):
@@ -351,11 +371,14 @@ class Engine:
 
 
 def poisson_stream(rate, duration, seed=0, mix=(0.6, 0.3, 0.1), sizes=None, nsteps=500,
-                   newton_case=None, ra=(2500.0, 3000.0), ra_s=(300.0, 500.0), cont_points=20):
+                   newton_case=None, ra=(2500.0, 3000.0), ra_s=(300.0, 500.0), cont_points=20,
+                   arclength=False):
     """Jobs arriving as a Poisson process of `rate` per second over `duration`
     seconds.  Time integrations draw their size from `sizes` (N_r, N_fm);
     Newton and continuation jobs start from `newton_case` (a dict with the
-    parameters and a state near a steady solution)."""
+    parameters and a state near a steady solution).  Continuations have
+    `cont_points` points in all: natural steps of newton_case["dRa"] in Ra_T,
+    or with `arclength` pseudo-arclength steps starting at ds = dRa."""
     rng = np.random.default_rng(seed)
     sizes = sizes or [(r, k) for r in (20, 30, 40) for k in (64, 128, 256, 512)]
     t, jobs = 0.0, []
@@ -373,10 +396,14 @@ def poisson_stream(rate, duration, seed=0, mix=(0.6, 0.3, 0.1), sizes=None, nste
             c = newton_case
             kind = "newton" if u < mix[0] + mix[1] else "continuation"
             ra0 = c["Ra"] * (1.0 + 0.02 * rng.uniform(-1, 1))
-            path = None
+            path, arc = None, 0
             if kind == "continuation":
-                path = list(ra0 + c.get("dRa", 20.0) * np.arange(1, cont_points))
+                if arclength:
+                    arc = cont_points - 1
+                else:
+                    path = list(ra0 + c.get("dRa", 20.0) * np.arange(1, cont_points))
             jobs.append(Job(kind, c["N_fm"], c["N_r"], ra0, c["Ra_s"], c["X"], d=c["d"],
                             Pr=c["Pr"], Tau=c["Tau"], symmetric=c.get("symmetric", False),
-                            ra_path=path, restart=c.get("restart", 100), arrival=t))
+                            ra_path=path, arc_points=arc, ds=c.get("dRa", 20.0),
+                            restart=c.get("restart", 100), arrival=t))
     return jobs

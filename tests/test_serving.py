@@ -20,7 +20,10 @@ def test_poisson_stream_mix_and_sizes():
     assert abs(frac["time"] - 0.6) < 0.1 and abs(frac["newton"] - 0.3) < 0.1
     assert all(np.diff([j.arrival for j in jobs]) > 0)
     cont = [j for j in jobs if j.kind == "continuation"]
-    assert cont and all(len(j.ra_path) == 19 for j in cont)
+    assert cont and all(len(j.ra_path) == 19 and j.arc_points == 0 for j in cont)
+    arc = serving.poisson_stream(200.0, 2.0, seed=1, newton_case=case, sizes=[(12, 16)],
+                                 arclength=True)
+    assert all(j.arc_points == 19 and j.ra_path is None for j in arc if j.kind == "continuation")
     assert serving.class_bytes(512, 40, 64, "mixed") > serving.class_bytes(64, 20, 64, "mixed")
 
 
@@ -48,6 +51,8 @@ def test_jobs_same_alone_and_in_the_mix():
                                 Tau=1.0, symmetric=True, restart=60, arrival=2.0 + i))
     jobs.append(serving.Job("continuation", Kw, Nrw, 6780.0, 0.0, X, d=d, Pr=10.0, Tau=1.0,
                             symmetric=True, restart=60, ra_path=[6800.0, 6820.0], arrival=3.0))
+    jobs.append(serving.Job("continuation", Kw, Nrw, 6780.0, 0.0, X, d=d, Pr=10.0, Tau=1.0,
+                            symmetric=True, restart=60, arc_points=3, ds=10.0, arrival=3.5))
     eng = serving.Engine(precision="fp64", capacity=4, chunk=7, newton_workers=2)
     done = eng.run(jobs, realtime=False)
     eng.shutdown()
@@ -76,11 +81,25 @@ def test_jobs_same_alone_and_in_the_mix():
             Xa, info = kr.newton(torch.as_tensor(j.X0, device="cuda"), j.Ra, j.Ra_s)
             assert info == j.info, (info, j.info)
             assert np.array_equal(Xa.cpu().numpy(), j.result)
-        else:
+        elif j.arc_points == 0:
             Xa = torch.as_tensor(j.X0, device="cuda")
             for (ra, Xm), ji in zip(j.result, j.info):
                 Xa, info = kr.newton(Xa, ra, j.Ra_s)
                 assert info == ji, (info, ji)
                 assert np.array_equal(Xa.cpu().numpy(), Xm)
+        else:
+            # Newton at Ra, then arclength steps: the same bits alone, and Ra_T
+            # increases along the branch
+            assert len(j.result) == 1 + j.arc_points
+            Xa, info = kr.newton(torch.as_tensor(j.X0, device="cuda"), j.Ra, j.Ra_s)
+            assert info == j.info[0] and np.array_equal(Xa.cpu().numpy(), j.result[0][1])
+            Y = torch.cat((Xa, torch.tensor([j.Ra], dtype=torch.float64, device="cuda")))
+            Yd, ds = torch.zeros_like(Y), j.ds
+            for (ra, Xm), ji in zip(j.result[1:], j.info[1:]):
+                Y, Yd, ds, info = kr.arclength_step(Y, Yd, j.Ra_s, j.sign, ds)
+                assert info == ji and info["status"] == 0, (info, ji)
+                assert np.array_equal(Y.cpu().numpy()[:-1], Xm) and Y[-1].item() == ra
+            ras = [ra for ra, _ in j.result]
+            assert all(np.diff(ras) > 0), ras
     first = [j for j in jobs if j.kind == "newton"][0]
     assert first.info["converged"], first.info   # from its own steady state
