@@ -400,4 +400,89 @@ __device__ cplx<R>* fft_run_w(cplx<R>* a, cplx<R>* b, const FftShape& sh,
   return a;
 }
 
+// ---------------------------------------------------------------------------
+// In-place wavefront FFT.  Each pass reads its whole array into registers
+// (one chunk of kIpChunk<RADIX> butterflies per lane covers all L/RADIX) before
+// it writes, so input and output share one buffer: half the LDS of the
+// ping-pong form.  LDS instructions of a wavefront issue in program order
+// and the buffer is not restrict-qualified, so the compiler keeps every read
+// of a pass ahead of its writes.  ip_shape_ok checks the chunk condition.
+// ---------------------------------------------------------------------------
+template <int RADIX>
+struct IpChunk {
+  static constexpr int U = RADIX >= 5 ? 2 : (RADIX == 4 ? 3 : 4);
+};
+
+inline bool ip_shape_ok(const FftShape& sh) {
+  if (sh.npass <= 0) return false;
+  for (int q = 0; q < sh.npass; ++q) {
+    const int r = sh.radix[q];
+    int U = 0;
+    if (r == 2) U = IpChunk<2>::U;
+    if (r == 3) U = IpChunk<3>::U;
+    if (r == 4) U = IpChunk<4>::U;
+    if (r == 5) U = IpChunk<5>::U;
+    if (U == 0 || sh.L / r > 64 * U) return false;
+  }
+  return true;
+}
+
+template <typename R, int RADIX>
+__device__ __forceinline__ void fft_pass_ip(cplx<R>* buf, int L, int Ns,
+                                            const cplx<R>* __restrict__ tw, bool inv, int lane) {
+  constexpr int U = IpChunk<RADIX>::U;
+  const int nb = L / RADIX;
+  const int st = L / (Ns * RADIX);
+  const float rns = 1.0f / (float)Ns;
+  cplx<R> v[U][RADIX], w[U][RADIX];
+  int base[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int jr = lane + 64 * u;
+    const int j = jr < nb ? jr : nb - 1;
+    const int jq = qdiv(j, Ns, rns);
+    const int k = j - jq * Ns;
+    base[u] = jq * Ns * RADIX + k;
+#pragma unroll
+    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[j + s * nb];
+#pragma unroll
+    for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, s * k * st, inv);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+#pragma unroll
+    for (int s = 1; s < RADIX; ++s) v[u][s] = cmul(v[u][s], w[u][s]);
+    if (RADIX == 2) dft2(v[u]);
+    if (RADIX == 3) dft3(v[u], inv);
+    if (RADIX == 4) dft4(v[u], inv);
+    if (RADIX == 5) dft5(v[u], inv);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (lane + 64 * u < nb)
+#pragma unroll
+      for (int s = 0; s < RADIX; ++s) buf[base[u] + s * Ns] = v[u][s];
+}
+
+// All passes of one transform in place by one wavefront (shape checked by
+// ip_shape_ok on the host).
+template <typename R>
+__device__ void fft_run_ip(cplx<R>* buf, const FftShape& sh, const cplx<R>* __restrict__ tw,
+                           bool inv, int lane) {
+  int Ns = 1;
+  for (int p = 0; p < sh.npass; ++p) {
+    const int r = sh.radix[p];
+    if (r == 4)
+      fft_pass_ip<R, 4>(buf, sh.L, Ns, tw, inv, lane);
+    else if (r == 2)
+      fft_pass_ip<R, 2>(buf, sh.L, Ns, tw, inv, lane);
+    else if (r == 3)
+      fft_pass_ip<R, 3>(buf, sh.L, Ns, tw, inv, lane);
+    else
+      fft_pass_ip<R, 5>(buf, sh.L, Ns, tw, inv, lane);
+    wave_lds_sync();
+    Ns *= r;
+  }
+}
+
 }  // namespace sddc

@@ -42,7 +42,7 @@
 
 namespace sddc {
 
-constexpr int kDepth = 4;    // chain rows of operator data in flight per lane
+constexpr int kDepth = 2;    // chain rows of operator data in flight per lane
 constexpr int kDepthE = 2;   // segment exit maps in flight per lane (join)
 
 struct ChainArgs {

@@ -602,6 +602,219 @@ __global__ __launch_bounds__(320) void k_nonlinear_w(FftShape sh, const cplx<R>*
   }
 }
 
+// ---------------------------------------------------------------------------
+// F(X, X) with in-place wavefront FFTs, for grids of L = 64*NC points
+// (N_fm = 2L/3 = 64*NK modes).  Same work split and arithmetic as
+// k_nonlinear_w, with three changes that cut LDS from 12 L to 8 L complex
+// values (three workgroups per CU at L = 768 in float):
+//  * each transform runs in place in one buffer (fft_run_ip);
+//  * the slab rows are not staged in LDS: lane l holds modes l + 64j in
+//    registers, the mirrored coefficient at index L - k needed by the
+//    Makhoul packing comes from lane (64 - l) mod 64 by a cross-lane read;
+//  * the half-sample rotations are read from global memory (issued with the
+//    slab loads), not kept in LDS.
+// LDS: twiddles (L), five pair buffers (5 L), two product buffers (2 L).
+// ---------------------------------------------------------------------------
+template <typename R, int NC>
+__global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
+                                                      const cplx<R>* __restrict__ rot_g, int nr,
+                                                      const R* __restrict__ slabs,
+                                                      R* __restrict__ F) {
+  constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
+  static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ R sinv[10];   // unscaling factors of the nine fields
+  using C = cplx<R>;
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* buf = tw + L;
+  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int i = blockIdx.x, b = blockIdx.y;
+  const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
+  const R *u, *v = nullptr;
+  bool us, vs = false;
+  int um, vm = 0;
+  field_src(sg, 2 * p, u, us, um);
+  if (p < 4) field_src(sg, 2 * p + 1, v, vs, vm);
+  // every global load of the first phase is issued before any is used
+  constexpr int NT = (L + 319) / 320;
+  C tws[NT], rr[NC];
+  R du[NK], dv[NK];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int kr = threadIdx.x + 320 * j;
+    tws[j] = tw_g[kr < L ? kr : L - 1];
+  }
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    du[j] = u[lane + 64 * j];
+    dv[j] = v ? v[lane + 64 * j] : (R)0;
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) rr[c] = rot_g[lane + 64 * c];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+    if (threadIdx.x + 320 * j < L) tw[threadIdx.x + 320 * j] = tws[j];
+  // mode factors (Derivatives, Matrix_Operators.py:709-729) and exponents
+  int eu = -1000, ev = -1000;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    const int k = lane + 64 * j;
+    R a = du[j], c = dv[j];
+    if (um == 1) a = (R)k * a;
+    if (um == 2) a = (R)(-k) * a;
+    if (vm == 1) c = (R)k * c;
+    if (vm == 2) c = (R)(-k) * c;
+    if (!(us && k == 0)) eu = max(eu, bexp(a));
+    if (!(vs && k == 0)) ev = max(ev, bexp(c));
+    du[j] = a;
+    dv[j] = c;
+  }
+  eu = __ockl_wfred_max_i32(eu);
+  ev = __ockl_wfred_max_i32(ev);
+  const R su = pow2_scale_e<R>(eu), sv = pow2_scale_e<R>(ev);
+  if (lane == 0) {
+    sinv[2 * p] = (R)1 / su;
+    if (p < 4) sinv[2 * p + 1] = (R)1 / sv;
+  }
+  // scaled synthesis coefficients a''_k at mode k = lane + 64 j (see syn_coef;
+  // a sine row's index 0 is never used)
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    const bool k0 = (lane == 0 && j == 0);
+    du[j] = su * (k0 ? (us ? (R)0 : du[j]) : (R)0.5 * du[j]);
+    dv[j] = sv * (k0 ? (vs ? (R)0 : dv[j]) : (R)0.5 * dv[j]);
+  }
+  // mirrored coefficients at index L - (lane + 64 c): lane (64 - lane) mod 64,
+  // register NC-1-c (lane 0: its own register NC-c)
+  R mu[NC], mv[NC];
+  const int src = (64 - lane) & 63;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    R xu = (R)0, xv = (R)0;
+    if (NC - 1 - c < NK) {
+      xu = __shfl(du[NC - 1 - c], src);
+      xv = __shfl(dv[NC - 1 - c], src);
+    }
+    if (lane == 0) {
+      xu = (NC - c < NK) ? du[NC - c < NK ? NC - c : 0] : (R)0;
+      xv = (NC - c < NK) ? dv[NC - c < NK ? NC - c : 0] : (R)0;
+    }
+    mu[c] = xu;
+    mv[c] = xv;
+  }
+  // packed, rotated spectrum W_k (Makhoul) into this pair's buffer
+  C* A = buf + (size_t)p * L;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int k = lane + 64 * c;
+    const R dcu = c < NK ? du[c < NK ? c : 0] : (R)0;
+    const R dcv = c < NK ? dv[c < NK ? c : 0] : (R)0;
+    const R au = us ? mu[c] : dcu, av = vs ? mv[c] : dcv;
+    const R aum = k ? (us ? dcu : mu[c]) : (R)0;
+    const R avm = k ? (vs ? dcv : mv[c]) : (R)0;
+    A[k] = cmul(C{au + avm, av - aum}, rr[c]);
+  }
+  __syncthreads();   // twiddles in LDS
+  fft_run_ip<R>(A, sh, tw, true, lane);
+  __syncthreads();
+  if (p >= 2) return;
+
+  // products of the grid fields (Matrix_Operators.py:791-793), then the
+  // forward transform of this wave's pair in its own buffer
+  const C* G0 = buf;
+  const C* G1 = buf + L;
+  const C* G2 = buf + 2 * L;
+  const C* G3 = buf + 3 * L;
+  const C* G4 = buf + 4 * L;
+  C* O = buf + (size_t)(5 + p) * L;
+  C er[NK];
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    const int k = lane + 64 * j;
+    const int ka = (p == 0) ? L - k : k;   // sine coefficients read index L-k
+    er[j] = rot_g[ka < L ? ka : ka - L];
+  }
+  R is[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) is[f] = sinv[f];
+  C o[NC];
+  int e0 = -1000, e1 = -1000;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int q = lane + 64 * c;
+    const R gs = (R)grid_sign(q, L);
+    const C w0 = G0[q], w1 = G1[q], w2 = G2[q];
+    const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
+    const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
+    if (p == 0) {
+      // Matrix_Operators.py:791 (Dr applied to JT*omega after projection)
+      const R kDpsi = w2.x * is[4];
+      o[c] = {gs * (JT * Om), gs * (kDpsi * Om + Dpsi * kOm)};
+    } else {
+      // Matrix_Operators.py:792-793
+      const C w3 = G3[q], w4 = G4[q];
+      const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
+              DC = w4.x * is[8];
+      o[c] = {JT * DT - Dpsi * kT, JT * DC - Dpsi * kC};
+    }
+    e0 = max(e0, bexp(o[c].x));
+    e1 = max(e1, bexp(o[c].y));
+  }
+  e0 = __ockl_wfred_max_i32(e0);
+  e1 = __ockl_wfred_max_i32(e1);
+  const R s0 = pow2_scale_e<R>(e0), s1 = pow2_scale_e<R>(e1);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) O[lane + 64 * c] = {o[c].x * s0, o[c].y * s1};
+  wave_lds_sync();
+  fft_run_ip<R>(O, sh, tw, false, lane);
+  const R i0 = (R)1 / s0, i1 = (R)1 / s1, invL = (R)1 / (R)L;
+  R* f0 = F + ((size_t)(b * 4 + 2 * p) * nr + i) * K;
+  R* f1 = F + ((size_t)(b * 4 + 2 * p + 1) * nr + i) * K;
+  C za[NK], zb[NK];
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    const int k = lane + 64 * j;
+    const int ka = (p == 0) ? L - k : k;
+    const int kam = ka < L ? ka : ka - L;
+    const int kb = (L - ka) < L ? L - ka : L - ka - L;
+    za[j] = O[kam];
+    zb[j] = O[kb];
+  }
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    const int k = lane + 64 * j;
+    // X = (a + conj b)/2, Y = (a - conj b)/(2i); C = Re(conj(e) X), Re(conj(e) Y)
+    const R xr = (R)0.5 * (za[j].x + zb[j].x), xi = (R)0.5 * (za[j].y - zb[j].y);
+    const R yr = (R)0.5 * (za[j].y + zb[j].y), yi = (R)-0.5 * (za[j].x - zb[j].x);
+    const R c1 = er[j].x * xr + er[j].y * xi;
+    const R c2 = er[j].x * yr + er[j].y * yi;
+    R a1, a2;
+    if (p == 0) {   // sine coefficients (k = 0 is zero)
+      a1 = k > 0 ? ((R)2 * (c1 * i0)) * invL : (R)0;
+      a2 = k > 0 ? ((R)2 * (c2 * i1)) * invL : (R)0;
+    } else {        // cosine coefficients
+      a1 = ((R)2 * (c1 * i0)) * invL;
+      a2 = ((R)2 * (c2 * i1)) * invL;
+      if (k == 0) {
+        a1 *= (R)0.5;
+        a2 *= (R)0.5;
+      }
+    }
+    f0[k] = a1;
+    f1[k] = a2;
+  }
+}
+
+// Columns per lane NC of the in-place kernel for this plan, 0 when its
+// grid does not fit it (k_nonlinear_w is used then).
+static int nl_inplace_nc(const Plan& p) {
+  const int L = p.tabM.shape.L;
+  if (L % 64 != 0 || 3 * p.K != 2 * L) return 0;
+  const int nc = L / 64;
+  if (nc != 3 && nc != 6 && nc != 12) return 0;
+  return ip_shape_ok(p.tabM.shape) ? nc : 0;
+}
+
 int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current) {
   if (bytes <= 64 * 1024 || bytes <= current) return SDDC_OK;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -615,7 +828,20 @@ int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current) {
 
 int nonlinear_lds_bytes(const Plan& p, int precision, int /*dfx*/) {
   const size_t c = precision == SDDC_MIXED ? sizeof(float2) : sizeof(double2);
-  return (int)(12 * (size_t)p.M * c);   // 2 x 5 working arrays + twiddles + rotations
+  // in place: twiddles + 5 pair + 2 product buffers; else 2 x 5 working
+  // arrays + twiddles + rotations
+  return (int)((nl_inplace_nc(p) ? 8 : 12) * (size_t)p.M * c);
+}
+
+template <typename R, int NC>
+static int launch_nl_ip(const Plan& p, int B, const void* slabs, void* F, hipStream_t s,
+                        const cplx<R>* tw, const cplx<R>* rot) {
+  const size_t lds = 8 * (size_t)(64 * NC) * sizeof(cplx<R>);
+  static size_t cur = 0;
+  if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_ip<R, NC>, lds, cur)) return rc;
+  hipLaunchKernelGGL((k_nonlinear_ip<R, NC>), dim3(p.nr, B), dim3(320), lds, s, p.tabM.shape, tw,
+                     rot, p.nr, (const R*)slabs, (R*)F);
+  return hip_check(hipGetLastError(), "k_nonlinear_ip");
 }
 
 template <typename R, bool DFX>
@@ -623,6 +849,12 @@ static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slab
                        hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
   const size_t lds = 12 * (size_t)p.M * sizeof(cplx<R>);
   if (!DFX) {
+    switch (nl_inplace_nc(p)) {
+      case 3: return launch_nl_ip<R, 3>(p, B, slabs, F, s, tw, rot);
+      case 6: return launch_nl_ip<R, 6>(p, B, slabs, F, s, tw, rot);
+      case 12: return launch_nl_ip<R, 12>(p, B, slabs, F, s, tw, rot);
+      default: break;
+    }
     if (lds + 64 > 160 * 1024)
       return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
     static size_t curw = 0;
