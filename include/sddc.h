@@ -71,11 +71,17 @@ typedef struct sddc_plan sddc_plan;
  *   chain_segment    rows per segment of the parity chains of the solves
  *                    (>= rows per chain: one segment, no join)
  *   solve_instances  instances per solve workgroup: 1, 2 or 4 (quad-lane
- *                    kernels), 16 (MFMA kernel; best for large batches) */
+ *                    kernels), 16 (MFMA kernel; best for large batches)
+ *   sub_batches      graphs only: the batch is split into this many parts
+ *                    (1-8), each stepped on its own forked stream inside the
+ *                    graph, so that the latency-bound solves of one part
+ *                    overlap the transforms of another.  Results are the
+ *                    same bits as one batch. */
 typedef struct sddc_tuning {
   int chain_segment;
   int solve_instances;
-  int reserved[6];
+  int sub_batches;
+  int reserved[5];
 } sddc_tuning;
 
 /* Plan lifecycle.  Replaces the operator tuple of Build_Matrix_Operators

@@ -38,7 +38,7 @@ class SddcOperators(ctypes.Structure):
 
 class SddcTuning(ctypes.Structure):
     _fields_ = [("chain_segment", ctypes.c_int), ("solve_instances", ctypes.c_int),
-                ("reserved", ctypes.c_int * 6)]
+                ("sub_batches", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
 
 
 class SddcKrylovOpts(ctypes.Structure):

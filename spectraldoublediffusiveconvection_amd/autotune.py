@@ -36,13 +36,16 @@ def bucket(B):
 
 def candidates(N_fm, B=1):
     """Launch shapes tried for a class with N_fm modes (N_fm/2 rows per chain)
-    at batch B: quad-lane solves with 1, 2 or 4 instances per workgroup, and
-    from B = 16 on the 16-instance MFMA solve."""
+    at batch B: quad-lane solves with 1, 2 or 4 instances per workgroup, from
+    B = 16 on the 16-instance MFMA solve, and from B = 64 on batches split in
+    2 or 4 parts on forked streams of the graph."""
     n = N_fm // 2
     segs = sorted({n, max(8, 4 * math.ceil(math.sqrt(n) / 4)),
                    max(8, 4 * math.ceil(2 * math.sqrt(n) / 4))})
     nbs = (1, 2, 4, 16) if B >= 16 else (1, 2, 4)
-    return [{"chain_segment": m, "solve_instances": nb} for m in segs for nb in nbs]
+    subs = (1, 2, 4) if B >= 64 else (1,)
+    return [{"chain_segment": m, "solve_instances": nb, "sub_batches": sb}
+            for m in segs for nb in nbs for sb in subs]
 
 
 def _key(N_fm, N_r, precision, B):
@@ -112,7 +115,8 @@ def tune_class(N_fm, N_r, B, precision="mixed", steps=10):
     best, best_t = None, float("inf")
     for t in candidates(N_fm, B):
         us = time_shape(N_fm, N_r, B, precision, t, steps=steps)
-        timings["seg%d_nb%d" % (t["chain_segment"], t["solve_instances"])] = round(us, 2)
+        timings["seg%d_nb%d_sb%d" % (t["chain_segment"], t["solve_instances"],
+                                     t["sub_batches"])] = round(us, 2)
         if us < best_t:
             best, best_t = t, us
     return best, timings

@@ -5,6 +5,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <utility>
 
 #include <vector>
 
@@ -190,6 +191,7 @@ int sddc_plan_tuning(const sddc_plan* plan, sddc_tuning* out) {
   *out = sddc_tuning{};
   out->chain_segment = plan->p.seg_m;
   out->solve_instances = plan->p.tune_solve_nbi;
+  out->sub_batches = plan->p.tune_sub_batches;
   return SDDC_OK;
 }
 
@@ -266,7 +268,10 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
     return bail(fail(SDDC_ERR_INVALID, "solve_instances must be 0, 1, 2, 4 or 16"));
   if (tuning && tuning->chain_segment < 0)
     return bail(fail(SDDC_ERR_INVALID, "chain_segment must be >= 0"));
+  if (tuning && (tuning->sub_batches < 0 || tuning->sub_batches > 8))
+    return bail(fail(SDDC_ERR_INVALID, "sub_batches must be in [0, 8]"));
   p.tune_solve_nbi = tuning ? tuning->solve_instances : 0;
+  p.tune_sub_batches = (tuning && tuning->sub_batches > 0) ? tuning->sub_batches : 1;
   if ((rc = build_chain(p, ops, max_batch > 0 ? max_batch : 1,
                         tuning ? tuning->chain_segment : 0)))
     return bail(rc);
@@ -445,14 +450,9 @@ static int step_once(Plan& p, int B, const double* Ra, const double* Ras, const 
   return launch_solve(p, B, p.G, dst, p.P, 7, 0, s);
 }
 
-int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s, const double* X_in,
-              double* X_out, int nsteps, int linear, void* stream) {
-  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
-  Plan& p = plan->p;
-  int rc = check_batch(p, B);
-  if (rc) return rc;
-  if (nsteps < 0) return fail(SDDC_ERR_INVALID, "negative step count");
-  hipStream_t s = (hipStream_t)stream;
+static int run_steps(Plan& p, int B, const double* Ra, const double* Ra_s, const double* X_in,
+                     double* X_out, int nsteps, int linear, hipStream_t s) {
+  int rc;
   if (nsteps == 0) {
     if (X_out != X_in)
       SDDC_HIP(hipMemcpyAsync(X_out, X_in, (size_t)B * 3 * p.K * p.nr * sizeof(double),
@@ -467,6 +467,55 @@ int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s, cons
     src = X_out;
   }
   return SDDC_OK;
+}
+
+// The plan restricted to instances [b0, b0 + nb): operators shared,
+// per-instance workspace offset by b0.  b0 is a multiple of 16, so the solve's
+// segment exit states stay aligned to whole workgroups.  Concurrent parts
+// share only the solve's sink (stores of padded rows, never read for real
+// instances).
+static Plan sub_plan(const Plan& p, int b0, int nb) {
+  Plan v = p;
+  const size_t N = (size_t)p.K * p.nr;
+  const size_t rs = p.precision == SDDC_MIXED ? sizeof(float) : sizeof(double);
+  v.P = p.P + b0 * N;
+  v.Pa = p.Pa + b0 * N;
+  v.L = p.L + b0 * 3 * N;
+  v.G = p.G + b0 * 3 * N;
+  v.Xa = p.Xa + b0 * 3 * N;
+  v.Xb = p.Xb + b0 * 3 * N;
+  v.slabs = static_cast<char*>(p.slabs) + b0 * 7 * N * rs;
+  v.slabs2 = static_cast<char*>(p.slabs2) + b0 * 7 * N * sizeof(double);
+  v.F = static_cast<char*>(p.F) + b0 * 4 * N * rs;
+  v.rowval = p.rowval + (size_t)b0 * p.nr;
+  v.chain_EX = p.chain_EX + (size_t)b0 * 6 * p.nseg * 2 * p.NRP;
+  v.ex_batch = p.ex_batch - b0;
+  v.max_batch = nb;
+  return v;
+}
+
+// Parts [b0, b0 + nb) of a batch of B for `parts` forked streams: sizes are
+// multiples of 16 (the last part takes the rest); fewer parts when B is small.
+static std::vector<std::pair<int, int>> split_batch(int B, int parts) {
+  std::vector<std::pair<int, int>> out;
+  if (parts < 2 || B < 32) {
+    out.push_back({0, B});
+    return out;
+  }
+  int q = (B + parts - 1) / parts;
+  q = ((q + 15) / 16) * 16;
+  for (int b0 = 0; b0 < B; b0 += q) out.push_back({b0, std::min(q, B - b0)});
+  return out;
+}
+
+int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s, const double* X_in,
+              double* X_out, int nsteps, int linear, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  int rc = check_batch(p, B);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(SDDC_ERR_INVALID, "negative step count");
+  return run_steps(p, B, Ra, Ra_s, X_in, X_out, nsteps, linear, (hipStream_t)stream);
 }
 
 int sddc_step_linearized(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
@@ -520,31 +569,13 @@ int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra
   int rc = check_batch(p, B);
   if (rc) return rc;
   if (nsteps < 1) return fail(SDDC_ERR_INVALID, "nsteps must be positive");
-  sddc_graph* g = new sddc_graph();
-  hipStream_t s = (hipStream_t)stream;
-  if (!s) {
-    if (hipStreamCreateWithFlags(&g->own, hipStreamNonBlocking) != hipSuccess) {
-      delete g;
-      return fail(SDDC_ERR_HIP, "stream creation failed");
-    }
-    s = g->own;
-  }
-  // run once eagerly so that every lazily set kernel attribute is in place
-  rc = sddc_step(plan, B, Ra, Ra_s, X, p.Xb, 1, 0, s);
-  if (!rc) rc = hip_check(hipStreamSynchronize(s), "warm-up");
-  if (!rc) rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture");
-  if (!rc) {
-    int rc2 = sddc_step(plan, B, Ra, Ra_s, X, X, nsteps, 0, s);
-    hipError_t e = hipStreamEndCapture(s, &g->graph);
-    rc = rc2 ? rc2 : hip_check(e, "end capture");
-  }
-  if (!rc) rc = hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "instantiate");
-  if (rc) {
-    sddc_graph_destroy(g);
-    return rc;
-  }
-  *out = g;
-  return SDDC_OK;
+  (void)p;
+  sddc_plan* const plans[1] = {plan};
+  const int Bs[1] = {B};
+  const double* const ras[1] = {Ra};
+  const double* const rass[1] = {Ra_s};
+  double* const xs[1] = {X};
+  return sddc_graph_create_multi(1, plans, Bs, ras, rass, xs, nsteps, stream, out);
 }
 
 int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
@@ -568,10 +599,20 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
       return bail(fail(SDDC_ERR_HIP, "stream creation failed"));
     s = g->own;
   }
-  g->side.assign(nplans, nullptr);
-  g->events.assign(nplans + 1, nullptr);
+  // one forked stream per part: each class split into its plan's sub_batches
+  struct Part {
+    Plan view;
+    int c, b0, nb;
+  };
+  std::vector<Part> parts;
   for (int c = 0; c < nplans; ++c)
-    if (hipStreamCreateWithFlags(&g->side[c], hipStreamNonBlocking) != hipSuccess)
+    for (auto [b0, nb] : split_batch(B[c], plans[c]->p.tune_sub_batches))
+      parts.push_back({sub_plan(plans[c]->p, b0, nb), c, b0, nb});
+  const int np = (int)parts.size();
+  g->side.assign(np, nullptr);
+  g->events.assign(np + 1, nullptr);
+  for (int q = 0; q < np; ++q)
+    if (hipStreamCreateWithFlags(&g->side[q], hipStreamNonBlocking) != hipSuccess)
       return bail(fail(SDDC_ERR_HIP, "stream creation failed"));
   for (auto& e : g->events)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
@@ -582,16 +623,21 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
     rc = sddc_step(plans[c], B[c], Ra[c], Ra_s[c], X[c], plans[c]->p.Xb, 1, 0, s);
   if (!rc) rc = hip_check(hipStreamSynchronize(s), "warm-up");
   if (rc) return bail(rc);
-  // fork: every class stream waits for the capture origin, runs its steps,
+  // fork: every part's stream waits for the capture origin, runs its steps,
   // and the origin waits for all of them
   if ((rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture")))
     return bail(rc);
-  int rc2 = hip_check(hipEventRecord(g->events[nplans], s), "fork");
-  for (int c = 0; c < nplans && !rc2; ++c) {
-    rc2 = hip_check(hipStreamWaitEvent(g->side[c], g->events[nplans], 0), "fork wait");
-    if (!rc2) rc2 = sddc_step(plans[c], B[c], Ra[c], Ra_s[c], X[c], X[c], nsteps, 0, g->side[c]);
-    if (!rc2) rc2 = hip_check(hipEventRecord(g->events[c], g->side[c]), "join");
-    if (!rc2) rc2 = hip_check(hipStreamWaitEvent(s, g->events[c], 0), "join wait");
+  int rc2 = hip_check(hipEventRecord(g->events[np], s), "fork");
+  for (int q = 0; q < np && !rc2; ++q) {
+    Part& pt = parts[q];
+    const size_t n3 = (size_t)3 * pt.view.K * pt.view.nr;
+    double* xq = X[pt.c] + pt.b0 * n3;
+    rc2 = hip_check(hipStreamWaitEvent(g->side[q], g->events[np], 0), "fork wait");
+    if (!rc2)
+      rc2 = run_steps(pt.view, pt.nb, Ra[pt.c] + pt.b0, Ra_s[pt.c] + pt.b0, xq, xq, nsteps, 0,
+                      g->side[q]);
+    if (!rc2) rc2 = hip_check(hipEventRecord(g->events[q], g->side[q]), "join");
+    if (!rc2) rc2 = hip_check(hipStreamWaitEvent(s, g->events[q], 0), "join wait");
   }
   hipError_t e = hipStreamEndCapture(s, &g->graph);
   rc = rc2 ? rc2 : hip_check(e, "end capture");

@@ -82,11 +82,11 @@ struct Plan {
   int NRS = 0;                 // nr rounded up to a multiple of 8
   int seg_m = 0, nseg = 0;     // rows per chain segment, segments per chain
   double* chain_L[3] = {nullptr, nullptr, nullptr};   // block inverses, K x NRP x NRS
-  double* chain_D2 = nullptr;                         // D2 of the A4 solve, NRP x NRS
+  double* chain_DL = nullptr;                         // D2 Linv_j of the A4 solve, per mode
   double* chain_Q[3] = {nullptr, nullptr, nullptr};   // row corrections (nseg > 1)
   double* chain_E[3] = {nullptr, nullptr, nullptr};   // segment exit maps (nseg > 1)
   double* chain_Lm[3] = {nullptr, nullptr, nullptr};  // MFMA-packed block inverses
-  double* chain_D2m = nullptr;                        // MFMA-packed D2 (A4)
+  double* chain_DLm = nullptr;                        // MFMA-packed D2 Linv_j (A4)
   TableSet tabM, tabKE;
   // workspace (device), sized for max_batch instances
   double* P = nullptr;
@@ -104,6 +104,7 @@ struct Plan {
   double* chain_EX = nullptr;        // zero-entry segment exit states
   int ex_batch = 0;                  // instances chain_EX holds
   int tune_solve_nbi = 0;            // instances per solve workgroup, 0 = automatic
+  int tune_sub_batches = 1;          // parts of a batch on forked streams in graphs
   size_t ws_bytes = 0;
 };
 

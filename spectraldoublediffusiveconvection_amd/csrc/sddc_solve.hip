@@ -50,11 +50,11 @@ struct ChainArgs {
   double* X;                // solutions, b*xstride + xoff[f]
   double* P;                // psi suffix sums [B][N] or null
   const double* Lq[3];      // packed block inverses per field, block index jj = K-1-row
-  const double* D2q;        // packed D2 of the A4 solve
+  const double* DLq;        // packed D2 Linv_j of the A4 solve, per mode
   const double* Qq[3];      // packed corrections, index jj
   const double* Eq[3];      // packed segment exit maps, index chain*nseg + seg
   const double* Lm[3];      // MFMA-packed block inverses (16-instance kernel)
-  const double* D2m;        // MFMA-packed D2 of the A4 solve
+  const double* DLm;        // MFMA-packed D2 Linv_j (16-instance kernel)
   const double* vecs;
   double* EX;               // zero-entry exit states [b][3][2][nseg][2*NRP]
   double* scratch;          // 2N + 256 doubles: sink of stores of padded instances/rows
@@ -252,24 +252,24 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
   const double dta = a.dtf[0];
   const double ir4 = a.vecs[V_IR4A4 * S::NRP + rr];
   const double ir2 = a.vecs[V_IR2A4 * S::NRP + rr];
-  double2 d2[S::NQ];
-  load_blk<S::NQ>(d2, a.D2q + w * S::NQ * 128 + 2 * lane);
   const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lane;
+  const double* mb = a.DLq + w * S::NQ * 128 + 2 * lane;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
-  double2 blk[kDepth][S::NQ];
+  double2 blk[kDepth][S::NQ], mblk[kDepth][S::NQ];
   double gv[kDepth][NB];
-  auto fetch = [&](int k, double2 (&bq)[S::NQ], double (&g)[NB]) {
+  auto fetch = [&](int k, double2 (&bq)[S::NQ], double2 (&mq)[S::NQ], double (&g)[NB]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
     load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
+    load_blk<S::NQ>(mq, mb + (size_t)(K - 1 - j) * S::BLK);
 #pragma unroll
     for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
   };
 #pragma unroll
-  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], mblk[d], gv[d]);
   drain_loads();
-  double fe[NB], bfe[NB];
+  double fe[NB], bfe[NB], dfe[NB];
 #pragma unroll
-  for (int t = 0; t < NB; ++t) fe[t] = bfe[t] = 0.0;
+  for (int t = 0; t < NB; ++t) fe[t] = bfe[t] = dfe[t] = 0.0;
   for (int kb = k0; kb < k1; kb += kDepth) {
 #pragma unroll
     for (int d = 0; d < kDepth; ++d) {
@@ -279,7 +279,9 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
       // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt, j = row + 1:
       // b = dt bjt (L1 f_e + IR4 bf_e) - bjt IR2 f_e with L1 = D2 + bj IR4;
       // f = Linv (g + b); bf_e += bj f + bjt f_e; f_e += f.  f_e is also the
-      // same-parity suffix sum P of the row.
+      // same-parity suffix sum P of the row.  D2 f_e is carried as a state
+      // too: D2 f = (D2 Linv) (g + b) comes from the same right-hand side as
+      // f, so a row needs one exchange of one vector.
       const double jp = (double)(row + 1);
       const double bj = -jp * (jp + 1.0);
       const double bjt = -2.0 * jp;
@@ -287,27 +289,23 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
       for (int t = 0; t < NB; ++t) {
         double* pdst = (live && rv) ? in.p[t] + (size_t)row * nr + r : sink;
         *pdst = fe[t];
-        xs[0][t][wp] = fe[t];
-      }
-      lds_barrier();
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        const double d2fe = quad_sum(pdot<S::NQ>(d2, &xs[0][t][kq * S::CPL]));
-        const double l1fe = d2fe + bj * (ir4 * fe[t]);
+        const double l1fe = dfe[t] + bj * (ir4 * fe[t]);
         const double bb = (dta * bjt) * (l1fe + ir4 * bfe[t]) - bjt * (ir2 * fe[t]);
-        xs[1][t][wp] = gv[d][t] + bb;
+        xs[d & 1][t][wp] = gv[d][t] + bb;
       }
       lds_barrier();
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[1][t][kq * S::CPL]));
+        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[d & 1][t][kq * S::CPL]));
+        const double df = quad_sum(pdot<S::NQ>(mblk[d], &xs[d & 1][t][kq * S::CPL]));
         double* dst = (live && rv) ? in.x[t] + (size_t)row * nr + r : sink;
         *dst = f;
         const double nbfe = bfe[t] + bj * f + bjt * fe[t];
         bfe[t] = live ? nbfe : bfe[t];
         fe[t] = live ? fe[t] + f : fe[t];
+        dfe[t] = live ? dfe[t] + df : dfe[t];
       }
-      fetch(k + kDepth, blk[d], gv[d]);
+      fetch(k + kDepth, blk[d], mblk[d], gv[d]);
     }
   }
   const size_t off = exit_seg - a.EX;
@@ -597,6 +595,22 @@ __device__ __forceinline__ void mfma_block(const double2 (&a)[NRS / 8], const do
   }
 }
 
+// Two products of one right-hand side: acc += A xs, accd += M xs (each B
+// fragment read once; the two MFMA chains are independent).
+template <int NRS>
+__device__ __forceinline__ void mfma_block2(const double2 (&a)[NRS / 8], const double2 (&m)[NRS / 8],
+                                            const double* xs, int lane, v4d_t& acc, v4d_t& accd) {
+#pragma unroll
+  for (int kp = 0; kp < NRS / 8; ++kp) {
+    const double b0 = xs[(8 * kp + (lane >> 4)) * 16 + (lane & 15)];
+    const double b1 = xs[(8 * kp + 4 + (lane >> 4)) * 16 + (lane & 15)];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kp].x, b0, acc, 0, 0, 0);
+    accd = __builtin_amdgcn_mfma_f64_16x16x4f64(m[kp].x, b0, accd, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kp].y, b1, acc, 0, 0, 0);
+    accd = __builtin_amdgcn_mfma_f64_16x16x4f64(m[kp].y, b1, accd, 0, 0, 0);
+  }
+}
+
 template <int NRS>
 __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a) {
   using S = CShape<NRS>;
@@ -687,20 +701,25 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
     for (int v = 0; v < 4; ++v) *((okr[v] && valid) ? exb + rv[v] : sink) = bs[v];
     return;
   }
-  // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt
+  // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt; D2 f_e carried as a
+  // state (D2 f = (D2 Linv_j) rhs from the same right-hand side as f)
   const double dta = a.dtf[0];
-  double2 d2[NKP];
-  {
-    const double* p = a.D2m + (size_t)w * NKP * 128 + 2 * lane;
+  const double* mbase = a.DLm + (size_t)w * NKP * 128 + 2 * lane;
+  double2 mfr[kDepth][NKP];
+  auto fetchm = [&](int k, double2 (&mq)[NKP]) {
+    const int j = start - 2 * (k < k1 ? k : k1 - 1);
+    const double* p = mbase + (size_t)(K - 1 - j) * MBLK;
 #pragma unroll
-    for (int q = 0; q < NKP; ++q) d2[q] = *reinterpret_cast<const double2*>(p + q * 128);
-  }
-  double ir4[4], ir2[4], fe[4], bfe[4];
+    for (int q = 0; q < NKP; ++q) mq[q] = *reinterpret_cast<const double2*>(p + q * 128);
+  };
+#pragma unroll
+  for (int d = 0; d < kDepth; ++d) fetchm(k0 + d, mfr[d]);
+  double ir4[4], ir2[4], fe[4], bfe[4], dfe[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     ir4[v] = a.vecs[V_IR4A4 * S::NRP + rrv[v]];
     ir2[v] = a.vecs[V_IR2A4 * S::NRP + rrv[v]];
-    fe[v] = bfe[v] = 0.0;
+    fe[v] = bfe[v] = dfe[v] = 0.0;
   }
   drain_loads();
   for (int kb = k0; kb < k1; kb += kDepth) {
@@ -715,28 +734,23 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         *((live && okr[v] && valid) ? pp + (size_t)row * nr + rv[v] : sink) = fe[v];
-        xs[0][wpos[v]] = fe[v];
-      }
-      lds_barrier();
-      v4d_t l1 = {0.0, 0.0, 0.0, 0.0};
-      mfma_block<NRS>(d2, xs[0], lane, l1);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const double l1fe = l1[v] + bj * (ir4[v] * fe[v]);
+        const double l1fe = dfe[v] + bj * (ir4[v] * fe[v]);
         const double bb = (dta * bjt) * (l1fe + ir4[v] * bfe[v]) - bjt * (ir2[v] * fe[v]);
-        xs[1][wpos[v]] = gv[d][v] + bb;
+        xs[d & 1][wpos[v]] = gv[d][v] + bb;
       }
       lds_barrier();
-      v4d_t acc = {0.0, 0.0, 0.0, 0.0};
-      mfma_block<NRS>(blk[d], xs[1], lane, acc);
+      v4d_t acc = {0.0, 0.0, 0.0, 0.0}, accd = {0.0, 0.0, 0.0, 0.0};
+      mfma_block2<NRS>(blk[d], mfr[d], xs[d & 1], lane, acc, accd);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         *((live && okr[v] && valid) ? xp + (size_t)row * nr + rv[v] : sink) = acc[v];
         const double nbfe = bfe[v] + bj * acc[v] + bjt * fe[v];
         bfe[v] = live ? nbfe : bfe[v];
         fe[v] = live ? fe[v] + acc[v] : fe[v];
+        dfe[v] = live ? dfe[v] + accd[v] : dfe[v];
       }
       fetch(k + kDepth, blk[d], gv[d]);
+      fetchm(k + kDepth, mfr[d]);
     }
   }
 #pragma unroll
@@ -832,8 +846,8 @@ int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
     a.goff[f] = goff[f];
     a.xoff[f] = xoff[f];
   }
-  a.D2q = p.chain_D2;
-  a.D2m = p.chain_D2m;
+  a.DLq = p.chain_DL;
+  a.DLm = p.chain_DLm;
   a.vecs = p.vecs;
   a.EX = p.chain_EX;
   a.scratch = p.solve_scratch;
@@ -961,17 +975,21 @@ int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment) 
       pack_quad(inv[f] + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
     if (int rc = upload_vec(&p.chain_L[f], buf)) return rc;
   }
-  std::vector<double> d2(BLK);
-  pack_quad(ops->D2_A4, nr, nr, nr, NRP, NRS, d2.data());
-  if (int rc = upload_vec(&p.chain_D2, d2)) return rc;
+  // D2 Linv_j per mode (float64 on the host): the A4 chains carry D2 f_e
+  std::vector<double> dl((size_t)K * NN);
+  for (int jj = 0; jj < K; ++jj) matmul(ops->D2_A4, inv[0] + jj * NN, dl.data() + jj * NN, nr, nr, nr);
+  for (int jj = 0; jj < K; ++jj)
+    pack_quad(dl.data() + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
+  if (int rc = upload_vec(&p.chain_DL, buf)) return rc;
   if (p.tune_solve_nbi == 16) {   // operands of the MFMA chain kernel
     for (int f = 0; f < 3; ++f) {
       for (int jj = 0; jj < K; ++jj)
         pack_mfma(inv[f] + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
       if (int rc = upload_vec(&p.chain_Lm[f], buf)) return rc;
     }
-    pack_mfma(ops->D2_A4, nr, nr, nr, NRP, NRS, d2.data());
-    if (int rc = upload_vec(&p.chain_D2m, d2)) return rc;
+    for (int jj = 0; jj < K; ++jj)
+      pack_mfma(dl.data() + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
+    if (int rc = upload_vec(&p.chain_DLm, buf)) return rc;
   }
   if (p.nseg == 1) return SDDC_OK;   // one segment per chain: no joins
 
@@ -1058,14 +1076,14 @@ void free_chain(Plan& p) {
     if (p.chain_E[f]) (void)hipFree(p.chain_E[f]);
     p.chain_L[f] = p.chain_Q[f] = p.chain_E[f] = nullptr;
   }
-  if (p.chain_D2) (void)hipFree(p.chain_D2);
-  p.chain_D2 = nullptr;
+  if (p.chain_DL) (void)hipFree(p.chain_DL);
+  p.chain_DL = nullptr;
   for (int f = 0; f < 3; ++f) {
     if (p.chain_Lm[f]) (void)hipFree(p.chain_Lm[f]);
     p.chain_Lm[f] = nullptr;
   }
-  if (p.chain_D2m) (void)hipFree(p.chain_D2m);
-  p.chain_D2m = nullptr;
+  if (p.chain_DLm) (void)hipFree(p.chain_DLm);
+  p.chain_DLm = nullptr;
 }
 
 }  // namespace sddc

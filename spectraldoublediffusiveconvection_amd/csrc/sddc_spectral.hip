@@ -95,8 +95,13 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
   __shared__ double sP[kTile + 1][NRP];     // modes k0 .. k0+16
   __shared__ double sT[kTile + 1][NRP];
   __shared__ double sS[kTile + 1][NRP];
-  __shared__ double sC[5][kTile][NRP + 1];  // contraction results [c][mode][i]
-  __shared__ double sV[V_COUNT * NRP];      // radial vectors
+  // contraction results [mode][i]: Dr psi, Dr T, Dr S in the precision they
+  // are computed in (float in mixed precision: 3 workgroups fit per CU);
+  // D2r2 psi and D2s psi in float64 (overwritten by omega and J_theta psi)
+  __shared__ R sCr[3][kTile][NRP + 1];
+  __shared__ double sCd[2][kTile][NRP + 1];
+  constexpr int kVecs = V_GR + 1;           // IR2, IR4, DT0, RSQ, GR
+  __shared__ double sV[kVecs * NRP];        // radial vectors
   const int K = a.K, nr = a.nr;
   const int b = blockIdx.y;
   const int k0 = blockIdx.x * kTile;
@@ -113,8 +118,9 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
   const int col = lane & 15, kq = lane >> 4;
   const bool f64 = (c == 2) || std::is_same<R, double>::value;
   const int slot = (c == 1) ? 1 : (c == 2 ? 2 : 0);
+  // one register array for both precisions (a float tile is held exactly
+  // in doubles), so the two paths do not each reserve registers
   double a64[NT][NK];
-  float a32[NT][NK];
   if (f64) {
     const double* At = a.A64 + (size_t)slot * NT * NK * 64 + lane;
 #pragma unroll
@@ -126,7 +132,7 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
 #pragma unroll
     for (int it = 0; it < NT; ++it)
 #pragma unroll
-      for (int kk = 0; kk < NK; ++kk) a32[it][kk] = At[(it * NK + kk) * 64];
+      for (int kk = 0; kk < NK; ++kk) a64[it][kk] = (double)At[(it * NK + kk) * 64];
   }
 
   constexpr int kLoadIters = ((kTile + 2) * NRP + 319) / 320;
@@ -146,12 +152,20 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
       }
     }
   }
-  constexpr int kVecIters = (V_COUNT * NRP + 319) / 320;
+  constexpr int kVecIters = (kVecs * NRP + 319) / 320;
 #pragma unroll
   for (int it = 0; it < kVecIters; ++it) {
     const int e = threadIdx.x + 320 * it;
-    if (e < V_COUNT * NRP) sV[e] = a.vecs[e];
+    if (e < kVecs * NRP) sV[e] = a.vecs[e];
   }
+  auto put = [&](int cc, int rl, int i, double v) {
+    if (cc == 1)
+      sCd[0][rl][i] = v;
+    else if (cc == 2)
+      sCd[1][rl][i] = v;
+    else
+      sCr[cc == 0 ? 0 : cc - 2][rl][i] = (R)v;   // exact: a value of type R
+  };
   __syncthreads();
 
   // (nr x nr) @ (nr x 16 modes) on MFMA.  Both shapes take A[i][k] from lane
@@ -168,15 +182,16 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a64[it][kk], srow[4 * kk], acc, 0, 0, 0);
         // float64 16x16x4 output: lane l, register v holds row l/16 + 4v
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sC[c][col][16 * it + kq + 4 * v] = acc[v];
+        for (int v = 0; v < 4; ++v) put(c, col, 16 * it + kq + 4 * v, acc[v]);
       } else {
         v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kk = 0; kk < NK; ++kk)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a32[it][kk], (float)srow[4 * kk], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a64[it][kk], (float)srow[4 * kk], acc, 0,
+                                                     0, 0);
         // float32 16x16x4 output: lane l, register v holds row 4*(l/16) + v
 #pragma unroll
-        for (int v = 0; v < 4; ++v) sC[c][col][16 * it + 4 * kq + v] = (double)acc[v];
+        for (int v = 0; v < 4; ++v) put(c, col, 16 * it + 4 * kq + v, (double)acc[v]);
       }
     }
   }
@@ -201,8 +216,8 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
     const double j = r + 1.0;                        // sine wavenumber of row r
     const double t = (j + 1.0) * psi + 2.0 * sP[rl][i];
     // A2_SINE_R2 (Matrix_Operators.py:510) and A2_SINE (:230)
-    const double Om = sC[1][rl][i] - j * (vIR4[i] * t);
-    const double A2 = sC[2][rl][i] - (j * vIR2[i]) * t;
+    const double Om = sCd[0][rl][i] - j * (vIR4[i] * t);
+    const double A2 = sCd[1][rl][i] - (j * vIR2[i]) * t;
     // J_theta_RT (Matrix_Operators.py:449-455): b = inclusive same-parity sum from row r+1
     const double Q = sPsi[rl + 2][i] + sP[rl + 1][i];
     const double JP = (r == 0) ? 1.0 * Q : (r + 1.0) * sPsi[rl][i] + 2.0 * Q;
@@ -225,8 +240,8 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
       Th[2 * N + o] = A2;
       Th[3 * N + o] = Om;
     }
-    sC[1][rl][i] = Om;   // same thread read these entries above
-    sC[2][rl][i] = JP;
+    sCd[0][rl][i] = Om;   // same thread read these entries above
+    sCd[1][rl][i] = JP;
   }
   __syncthreads();
 
@@ -238,14 +253,14 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
       const int r = k0 + rl;
       if (r >= K) continue;
       const size_t row = (size_t)i * K;
-      S7[row + r] = (R)sC[2][rl][i];
-      S7[3 * nK + row + r] = (R)sC[3][rl][i];
-      S7[4 * nK + row + r] = (R)sC[4][rl][i];
+      S7[row + r] = (R)sCd[1][rl][i];
+      S7[3 * nK + row + r] = (R)sCr[1][rl][i];
+      S7[4 * nK + row + r] = (R)sCr[2][rl][i];
       S7[5 * nK + row + r] = (R)sT[rl][i];
       S7[6 * nK + row + r] = (R)sS[rl][i];
       if (r + 1 < K) {   // sine rows move to the sinusoid index r+1 (Matrix_Operators.py:734-737)
-        S7[nK + row + r + 1] = (R)sC[0][rl][i];
-        S7[2 * nK + row + r + 1] = (R)sC[1][rl][i];
+        S7[nK + row + r + 1] = (R)sCr[0][rl][i];
+        S7[2 * nK + row + r + 1] = (R)sCd[0][rl][i];
       }
       if (r == 0) {
         S7[nK + row] = (R)0;
@@ -262,17 +277,17 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
       if (r >= K) continue;
       const size_t row = (size_t)i * K;
       // order: JT, kDpsi, komega, DT, DC, omega, Dpsi, kT, kC (Matrix_Operators.py:740)
-      S9[row + r] = sC[2][rl][i];
-      S9[3 * nK + row + r] = sC[3][rl][i];
-      S9[4 * nK + row + r] = sC[4][rl][i];
+      S9[row + r] = sCd[1][rl][i];
+      S9[3 * nK + row + r] = sCr[1][rl][i];
+      S9[4 * nK + row + r] = sCr[2][rl][i];
       S9[7 * nK + row + r] = (double)(-r) * sT[rl][i];
       S9[8 * nK + row + r] = (double)(-r) * sS[rl][i];
       if (r + 1 < K) {
         const double ks = r + 1.0;
-        S9[1 * nK + row + r + 1] = ks * sC[0][rl][i];
-        S9[2 * nK + row + r + 1] = ks * sC[1][rl][i];
-        S9[5 * nK + row + r + 1] = sC[1][rl][i];
-        S9[6 * nK + row + r + 1] = sC[0][rl][i];
+        S9[1 * nK + row + r + 1] = ks * sCr[0][rl][i];
+        S9[2 * nK + row + r + 1] = ks * sCd[0][rl][i];
+        S9[5 * nK + row + r + 1] = sCd[0][rl][i];
+        S9[6 * nK + row + r + 1] = sCr[0][rl][i];
       }
       if (r == 0) {
         S9[1 * nK + row] = 0.0;

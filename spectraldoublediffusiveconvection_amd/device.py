@@ -67,6 +67,7 @@ class Plan:
             tun = _lib.SddcTuning()
             tun.chain_segment = int(tuning.get("chain_segment", 0))
             tun.solve_instances = int(tuning.get("solve_instances", 0))
+            tun.sub_batches = int(tuning.get("sub_batches", 0))
         _lib.check(L.sddc_plan_create_tuned(ctypes.byref(s), self.precision, int(self.symmetric),
                                             int(max_batch), ctypes.byref(tun) if tun else None,
                                             ctypes.byref(h)))
@@ -80,10 +81,12 @@ class Plan:
         return cls(_ops.build(N_fm, N_r, d, dt, Pr, Tau), symmetric, precision, max_batch, tuning)
 
     def tuning(self):
-        """Launch shapes in use: {'chain_segment': rows, 'solve_instances': n (0 = by batch)}."""
+        """Launch shapes in use: {'chain_segment': rows, 'solve_instances': n
+        (0 = by batch), 'sub_batches': parts of a batch on forked streams in graphs}."""
         t = _lib.SddcTuning()
         _lib.check(_lib.lib().sddc_plan_tuning(self._h, ctypes.byref(t)))
-        return {"chain_segment": t.chain_segment, "solve_instances": t.solve_instances}
+        return {"chain_segment": t.chain_segment, "solve_instances": t.solve_instances,
+                "sub_batches": t.sub_batches}
 
     def close(self):
         if getattr(self, "_h", None):

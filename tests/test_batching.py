@@ -31,7 +31,7 @@ def test_sweep_graph_matches_eager_and_alone():
 
     from spectraldoublediffusiveconvection_amd import batching, device as dev, operators as ops
     insts = batching.random_sweep(9, seed=3, N_r=(12, 20), N_fm=(16, 64))
-    tuning = {"chain_segment": 8, "solve_instances": 2}
+    tuning = {"chain_segment": 8, "solve_instances": 2, "sub_batches": 2}
 
     class Fixed:
         def lookup(self, key, B, precision):
@@ -77,3 +77,30 @@ def test_tuned_plans_agree_with_oracle():
         y = p.step(torch.as_tensor(X, device="cuda"), 3000.0, 500.0).cpu().numpy()
         for a, b in zip(y, ref):
             assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-12, t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sub", [2, 4])
+def test_sub_batch_graph_same_bits(sub):
+    """A graph whose batch is split into parts on forked streams gives the
+    same bits as eager steps of the whole batch (parts of 16-instance
+    multiples and a ragged last part)."""
+    import torch
+
+    from spectraldoublediffusiveconvection_amd import device as dev, operators as ops
+    K, Nr, B = 64, 20, 70
+    p = dev.Plan(ops.build(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0), precision="mixed", max_batch=B,
+                 tuning={"sub_batches": sub})
+    assert p.tuning()["sub_batches"] == sub
+    rng = np.random.default_rng(sub)
+    X0 = 1e-3 * rng.standard_normal((B, 3 * K * (Nr - 1)))
+    Ra = torch.as_tensor(rng.uniform(2500, 3000, B), device="cuda")
+    xg = torch.as_tensor(X0, device="cuda")
+    g = p.graph(xg, Ra, 500.0, nsteps=3)
+    eager = torch.as_tensor(X0, device="cuda")
+    for rep in range(2):
+        g.launch()
+        eager = p.step(eager, Ra, 500.0, nsteps=3)
+        torch.cuda.synchronize()
+        assert torch.equal(xg, eager), rep
+    g.close()
