@@ -201,6 +201,13 @@ def main():
     el64 = time.perf_counter() - t1
     plain = n64 / el64
 
+    # ---- kinetic energy of the mixed method against float64 after 100 steps
+    p32 = dev.Plan(op, precision="mixed")
+    xa = torch.as_tensor(seeded_states(1, n3, seed=5), device="cuda")
+    k64 = p64.diagnostics(p64.step(xa, w["Ra_hi"], w["Ra_s"], nsteps=100))[0, 1].item()
+    k32 = p64.diagnostics(p32.step(xa, w["Ra_hi"], w["Ra_s"], nsteps=100))[0, 1].item()
+    ke_err = abs(k32 / k64 - 1.0)
+
     line = {
         "metric": "aggregate CN/AB2 steps/s at Nr=40,Ntheta=512 vs plain FP64 GPU; p50/p99 job latency",
         "value": value,
@@ -220,6 +227,7 @@ def main():
                    "launch_shape": plan.tuning()},
         "plain_fp64_gpu": {"steps_per_s": plain, "ms_per_step": 1e3 * el64 / n64},
         "speedup_vs_plain_fp64": value / plain,
+        "mixed_ke_rel_error_100_steps": ke_err,
         "roofline": roof,
     }
     if not args.no_sweep:

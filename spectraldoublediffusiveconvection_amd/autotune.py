@@ -40,8 +40,10 @@ def candidates(N_fm, B=1):
     B = 16 on the 16-instance MFMA solve, and from B = 64 on batches split in
     2 or 4 parts on forked streams of the graph."""
     n = N_fm // 2
-    segs = sorted({n, max(8, 4 * math.ceil(math.sqrt(n) / 4)),
-                   max(8, 4 * math.ceil(2 * math.sqrt(n) / 4))})
+    segs = {n, max(8, 4 * math.ceil(2 * math.sqrt(n) / 4))}
+    if B < 64:   # short segments only pay off while the device is not full
+        segs.add(max(8, 4 * math.ceil(math.sqrt(n) / 4)))
+    segs = sorted(segs)
     nbs = (1, 2, 4, 16) if B >= 16 else (1, 2, 4)
     subs = (1, 2, 4) if B >= 64 else (1,)
     return [{"chain_segment": m, "solve_instances": nb, "sub_batches": sb}

@@ -603,47 +603,34 @@ __global__ __launch_bounds__(320) void k_nonlinear_w(FftShape sh, const cplx<R>*
 }
 
 // ---------------------------------------------------------------------------
-// F(X, X) with in-place wavefront FFTs, for grids of L = 64*NC points
-// (N_fm = 2L/3 = 64*NK modes).  Same work split and arithmetic as
-// k_nonlinear_w, with three changes that cut LDS from 12 L to 8 L complex
-// values (three workgroups per CU at L = 768 in float):
+// In-place nonlinear kernels, for grids of L = 64*NC points (N_fm = 2L/3 =
+// 64*NK modes).  Same work split and arithmetic as k_nonlinear_w, with three
+// changes that cut LDS from 12 L to 8 L complex values for F(X, X) (three
+// workgroups per CU at L = 768 in float):
 //  * each transform runs in place in one buffer (fft_run_ip);
 //  * the slab rows are not staged in LDS: lane l holds modes l + 64j in
 //    registers, the mirrored coefficient at index L - k needed by the
 //    Makhoul packing comes from lane (64 - l) mod 64 by a cross-lane read;
 //  * the half-sample rotations are read from global memory (issued with the
 //    slab loads), not kept in LDS.
-// LDS: twiddles (L), five pair buffers (5 L), two product buffers (2 L).
 // ---------------------------------------------------------------------------
+
+// Loads field pair p of a slab set (one wavefront), scales both series by a
+// power of two and writes their packed, rotated spectrum W_k into A.  sinv
+// receives the two unscaling factors.
 template <typename R, int NC>
-__global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
-                                                      const cplx<R>* __restrict__ rot_g, int nr,
-                                                      const R* __restrict__ slabs,
-                                                      R* __restrict__ F) {
-  constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
-  static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  __shared__ R sinv[10];   // unscaling factors of the nine fields
+__device__ __forceinline__ void ip_pack_pair(const SlabRows<R>& sg, int p, cplx<R>* A,
+                                             const cplx<R>* __restrict__ rot_g, R* sinv,
+                                             int lane) {
+  constexpr int NK = 2 * NC / 3;
   using C = cplx<R>;
-  C* tw = reinterpret_cast<C*>(smem_raw);
-  C* buf = tw + L;
-  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int i = blockIdx.x, b = blockIdx.y;
-  const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
   const R *u, *v = nullptr;
   bool us, vs = false;
   int um, vm = 0;
   field_src(sg, 2 * p, u, us, um);
   if (p < 4) field_src(sg, 2 * p + 1, v, vs, vm);
-  // every global load of the first phase is issued before any is used
-  constexpr int NT = (L + 319) / 320;
-  C tws[NT], rr[NC];
+  C rr[NC];
   R du[NK], dv[NK];
-#pragma unroll
-  for (int j = 0; j < NT; ++j) {
-    const int kr = threadIdx.x + 320 * j;
-    tws[j] = tw_g[kr < L ? kr : L - 1];
-  }
 #pragma unroll
   for (int j = 0; j < NK; ++j) {
     du[j] = u[lane + 64 * j];
@@ -651,9 +638,6 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   }
 #pragma unroll
   for (int c = 0; c < NC; ++c) rr[c] = rot_g[lane + 64 * c];
-#pragma unroll
-  for (int j = 0; j < NT; ++j)
-    if (threadIdx.x + 320 * j < L) tw[threadIdx.x + 320 * j] = tws[j];
   // mode factors (Derivatives, Matrix_Operators.py:709-729) and exponents
   int eu = -1000, ev = -1000;
 #pragma unroll
@@ -673,8 +657,8 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   ev = __ockl_wfred_max_i32(ev);
   const R su = pow2_scale_e<R>(eu), sv = pow2_scale_e<R>(ev);
   if (lane == 0) {
-    sinv[2 * p] = (R)1 / su;
-    if (p < 4) sinv[2 * p + 1] = (R)1 / sv;
+    sinv[0] = (R)1 / su;
+    if (p < 4) sinv[1] = (R)1 / sv;
   }
   // scaled synthesis coefficients a''_k at mode k = lane + 64 j (see syn_coef;
   // a sine row's index 0 is never used)
@@ -702,8 +686,7 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
     mu[c] = xu;
     mv[c] = xv;
   }
-  // packed, rotated spectrum W_k (Makhoul) into this pair's buffer
-  C* A = buf + (size_t)p * L;
+  // packed, rotated spectrum W_k (Makhoul)
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int k = lane + 64 * c;
@@ -714,19 +697,39 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
     const R avm = k ? (vs ? dcv : mv[c]) : (R)0;
     A[k] = cmul(C{au + avm, av - aum}, rr[c]);
   }
-  __syncthreads();   // twiddles in LDS
-  fft_run_ip<R>(A, sh, tw, true, lane);
-  __syncthreads();
-  if (p >= 2) return;
+}
 
-  // products of the grid fields (Matrix_Operators.py:791-793), then the
-  // forward transform of this wave's pair in its own buffer
-  const C* G0 = buf;
-  const C* G1 = buf + L;
-  const C* G2 = buf + 2 * L;
-  const C* G3 = buf + 3 * L;
-  const C* G4 = buf + 4 * L;
-  C* O = buf + (size_t)(5 + p) * L;
+// Twiddles of the workgroup's transforms: loaded into registers first (in
+// flight with the slab loads), stored to LDS after the packing.
+template <typename R, int NC, int NTHR>
+struct IpTwiddles {
+  static constexpr int L = 64 * NC, NT = (L + NTHR - 1) / NTHR;
+  cplx<R> t[NT];
+  __device__ __forceinline__ void load(const cplx<R>* __restrict__ tw_g) {
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int kr = threadIdx.x + NTHR * j;
+      t[j] = tw_g[kr < L ? kr : L - 1];
+    }
+  }
+  __device__ __forceinline__ void store(cplx<R>* tw) const {
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+      if (threadIdx.x + NTHR * j < L) tw[threadIdx.x + NTHR * j] = t[j];
+  }
+};
+
+// The second phase for wavefront p in {0, 1}: products o (registers) scaled
+// to unit magnitude, written to O, forward FFT in place, projection onto the
+// K modes and stores of output rows 2p, 2p+1.
+template <typename R, int NC>
+__device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O, int p,
+                                                   const cplx<R>* tw, const FftShape& sh,
+                                                   const cplx<R>* __restrict__ rot_g,
+                                                   R* __restrict__ F, int b, int i, int nr,
+                                                   int lane) {
+  constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
+  using C = cplx<R>;
   C er[NK];
 #pragma unroll
   for (int j = 0; j < NK; ++j) {
@@ -734,29 +737,9 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
     const int ka = (p == 0) ? L - k : k;   // sine coefficients read index L-k
     er[j] = rot_g[ka < L ? ka : ka - L];
   }
-  R is[9];
-#pragma unroll
-  for (int f = 0; f < 9; ++f) is[f] = sinv[f];
-  C o[NC];
   int e0 = -1000, e1 = -1000;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const int q = lane + 64 * c;
-    const R gs = (R)grid_sign(q, L);
-    const C w0 = G0[q], w1 = G1[q], w2 = G2[q];
-    const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
-    const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
-    if (p == 0) {
-      // Matrix_Operators.py:791 (Dr applied to JT*omega after projection)
-      const R kDpsi = w2.x * is[4];
-      o[c] = {gs * (JT * Om), gs * (kDpsi * Om + Dpsi * kOm)};
-    } else {
-      // Matrix_Operators.py:792-793
-      const C w3 = G3[q], w4 = G4[q];
-      const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
-              DC = w4.x * is[8];
-      o[c] = {JT * DT - Dpsi * kT, JT * DC - Dpsi * kC};
-    }
     e0 = max(e0, bexp(o[c].x));
     e1 = max(e1, bexp(o[c].y));
   }
@@ -805,6 +788,143 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   }
 }
 
+// F(X, X): five wavefronts, wavefront p synthesises field pair p.
+// LDS: twiddles (L), five pair buffers (5 L), two product buffers (2 L).
+template <typename R, int NC>
+__global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
+                                                      const cplx<R>* __restrict__ rot_g, int nr,
+                                                      const R* __restrict__ slabs,
+                                                      R* __restrict__ F) {
+  constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
+  static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ R sinv[10];   // unscaling factors of the nine fields
+  using C = cplx<R>;
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* buf = tw + L;
+  const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int i = blockIdx.x, b = blockIdx.y;
+  const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
+  IpTwiddles<R, NC, 320> twr;
+  twr.load(tw_g);
+  C* A = buf + (size_t)p * L;
+  ip_pack_pair<R, NC>(sg, p, A, rot_g, sinv + 2 * p, lane);
+  twr.store(tw);
+  __syncthreads();   // twiddles in LDS
+  fft_run_ip<R>(A, sh, tw, true, lane);
+  __syncthreads();
+  if (p >= 2) return;
+
+  // products of the grid fields (Matrix_Operators.py:791-793)
+  const C* G0 = buf;
+  const C* G1 = buf + L;
+  const C* G2 = buf + 2 * L;
+  const C* G3 = buf + 3 * L;
+  const C* G4 = buf + 4 * L;
+  R is[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) is[f] = sinv[f];
+  C o[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int q = lane + 64 * c;
+    const R gs = (R)grid_sign(q, L);
+    const C w0 = G0[q], w1 = G1[q], w2 = G2[q];
+    const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
+    const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
+    if (p == 0) {
+      // Matrix_Operators.py:791 (Dr applied to JT*omega after projection)
+      const R kDpsi = w2.x * is[4];
+      o[c] = {gs * (JT * Om), gs * (kDpsi * Om + Dpsi * kOm)};
+    } else {
+      // Matrix_Operators.py:792-793
+      const C w3 = G3[q], w4 = G4[q];
+      const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
+              DC = w4.x * is[8];
+      o[c] = {JT * DT - Dpsi * kT, JT * DC - Dpsi * kC};
+    }
+  }
+  ip_forward_project<R, NC>(o, buf + (size_t)(5 + p) * L, p, tw, sh, rot_g, F, b, i, nr, lane);
+}
+
+// F(X, dv) + F(dv, X) (NLIN_DFX, Matrix_Operators.py:807-898): ten
+// wavefronts, wavefront q < 5 synthesises pair q of X, q >= 5 pair q-5 of dv.
+// LDS: twiddles (L), ten pair buffers (10 L), two product buffers (2 L).
+template <typename R, int NC>
+__global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
+                                                          const cplx<R>* __restrict__ tw_g,
+                                                          const cplx<R>* __restrict__ rot_g,
+                                                          int nr, const R* __restrict__ slabs,
+                                                          const R* __restrict__ slabs2,
+                                                          R* __restrict__ F) {
+  constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
+  static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ R sinv[20];   // unscaling factors: fields of X, then of dv
+  using C = cplx<R>;
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* buf = tw + L;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int i = blockIdx.x, b = blockIdx.y;
+  const int p = q < 5 ? q : q - 5;
+  const SlabRows<R> sg = slab_rows<R>(q < 5 ? slabs : slabs2, b, i, nr, K);
+  IpTwiddles<R, NC, 640> twr;
+  twr.load(tw_g);
+  C* A = buf + (size_t)q * L;
+  ip_pack_pair<R, NC>(sg, p, A, rot_g, sinv + (q < 5 ? 0 : 10) + 2 * p, lane);
+  twr.store(tw);
+  __syncthreads();   // twiddles in LDS
+  fft_run_ip<R>(A, sh, tw, true, lane);
+  __syncthreads();
+  if (q >= 2) return;
+
+  // Frechet derivative (Matrix_Operators.py:884-887): every term is (field of
+  // X) * (field of dv), accumulated in the order of the pairs of dv
+  R is[9], id[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    is[f] = sinv[f];
+    id[f] = sinv[10 + f];
+  }
+  C o[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int qq = lane + 64 * c;
+    const R gs = (R)grid_sign(qq, L);
+    const C w0 = buf[qq], w1 = buf[L + qq], w2 = buf[2 * L + qq];
+    const C v0 = buf[5 * L + qq], v1 = buf[6 * L + qq], v2 = buf[7 * L + qq];
+    const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
+    const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
+    const R dJT = v0.x * id[0], dOm = gs * v0.y * id[1];
+    const R dkOm = v1.x * id[2], dDpsi = gs * v1.y * id[3];
+    const R dkDpsi = v2.x * id[4];
+    if (q == 0) {
+      const R kDpsi = w2.x * is[4];
+      R zx = JT * dOm + dJT * Om;
+      R zy = kDpsi * dOm;
+      zy += Dpsi * dkOm + dDpsi * kOm;
+      zy += dkDpsi * Om;
+      o[c] = {gs * zx, gs * zy};
+    } else {
+      const C w3 = buf[3 * L + qq], w4 = buf[4 * L + qq];
+      const C v3 = buf[8 * L + qq], v4 = buf[9 * L + qq];
+      const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
+              DC = w4.x * is[8];
+      const R dkT = gs * v2.y * id[5], dDT = v3.x * id[6], dkC = gs * v3.y * id[7],
+              dDC = v4.x * id[8];
+      R tx = dJT * DT, ty = dJT * DC;
+      tx -= dDpsi * kT;
+      ty -= dDpsi * kC;
+      tx -= Dpsi * dkT;
+      tx += JT * dDT;
+      ty -= Dpsi * dkC;
+      ty += JT * dDC;
+      o[c] = {tx, ty};
+    }
+  }
+  ip_forward_project<R, NC>(o, buf + (size_t)(10 + q) * L, q, tw, sh, rot_g, F, b, i, nr, lane);
+}
+
 // Columns per lane NC of the in-place kernel for this plan, 0 when its
 // grid does not fit it (k_nonlinear_w is used then).
 static int nl_inplace_nc(const Plan& p) {
@@ -844,6 +964,17 @@ static int launch_nl_ip(const Plan& p, int B, const void* slabs, void* F, hipStr
   return hip_check(hipGetLastError(), "k_nonlinear_ip");
 }
 
+template <typename R, int NC>
+static int launch_nl_dfx_ip(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
+                            hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
+  const size_t lds = 13 * (size_t)(64 * NC) * sizeof(cplx<R>);
+  static size_t cur = 0;
+  if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_dfx_ip<R, NC>, lds, cur)) return rc;
+  hipLaunchKernelGGL((k_nonlinear_dfx_ip<R, NC>), dim3(p.nr, B), dim3(640), lds, s, p.tabM.shape,
+                     tw, rot, p.nr, (const R*)slabs, (const R*)slabs2, (R*)F);
+  return hip_check(hipGetLastError(), "k_nonlinear_dfx_ip");
+}
+
 template <typename R, bool DFX>
 static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
                        hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
@@ -862,6 +993,12 @@ static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slab
     hipLaunchKernelGGL((k_nonlinear_w<R>), dim3(p.nr, B), dim3(320), lds, s, p.tabM.shape, tw,
                        rot, p.K, p.nr, (const R*)slabs, (R*)F);
     return hip_check(hipGetLastError(), "k_nonlinear_w");
+  }
+  switch ((13 * (size_t)p.M * sizeof(cplx<R>) <= 160 * 1024) ? nl_inplace_nc(p) : 0) {
+    case 3: return launch_nl_dfx_ip<R, 3>(p, B, slabs, slabs2, F, s, tw, rot);
+    case 6: return launch_nl_dfx_ip<R, 6>(p, B, slabs, slabs2, F, s, tw, rot);
+    case 12: return launch_nl_dfx_ip<R, 12>(p, B, slabs, slabs2, F, s, tw, rot);
+    default: break;
   }
   if (lds + 16 * 18 * sizeof(int) > 160 * 1024)
     return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");

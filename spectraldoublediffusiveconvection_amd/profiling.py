@@ -24,8 +24,9 @@ def stage_bytes(plan, B, stage):
         return B * 11 * N * rs                                 # 7 slab rows in, 4 rows out
     if stage == 3:
         return B * (4 * N * rs + 3 * N * 8 + 3 * N * 8)        # F, L in; G out
-    # solves: G in, X and P out, three stacks of block inverses once per launch
-    return B * 7 * N * 8 + 3 * K * nr * nr * 8
+    # solves: G in, X and P out; the block inverses of the three fields and
+    # D2 Linv_j of the A4 chains once per launch
+    return B * 7 * N * 8 + 4 * K * nr * nr * 8
 
 
 def time_stage(plan, X, Ra, Ra_s, stage, reps=20):

@@ -260,6 +260,12 @@ def test_step_other_sizes(K, Nr):
     assert rel(H(p.nlin_fx(T(X0)))[0], F) < 1e-12
     pm = dev.Plan(op, precision="mixed")
     assert rel(H(pm.nlin_fx(T(X0)))[0], F) < 2e-5
+    dv = 1e-3 * np.random.default_rng(K).standard_normal(3 * K * (Nr - 1))
+    D = O.nlin_dfx(dv, X0, o)
+    assert rel(H(p.nlin_dfx(T(dv), T(X0)))[0], D) < 1e-12
+    assert rel(H(pm.nlin_dfx(T(dv), T(X0)))[0], D) < 2e-5
+    assert rel(H(p.step_linearized(T(dv), T(X0), 9000.0, 500.0))[0],
+               O.step_linearized(dv, X0, o, 9000.0, 500.0)) < 1e-12
 
 
 def test_batched_equals_alone_bitwise(c1):
