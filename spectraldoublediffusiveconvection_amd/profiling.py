@@ -30,9 +30,12 @@ def stage_bytes(plan, B, stage):
 
 
 def time_stage(plan, X, Ra, Ra_s, stage, reps=20):
-    """Average duration (ms) of one stage, HIP events on the current stream."""
+    """Average duration (ms) of one stage, HIP events on the current stream.
+    Ra, Ra_s should already be device tensors (stage_table converts them), so
+    that the timed region holds the stage's kernels and nothing else."""
     for _ in range(2):
         plan.run_stage(X, Ra, Ra_s, stage)
+    torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -45,8 +48,9 @@ def time_stage(plan, X, Ra, Ra_s, stage, reps=20):
 
 def stage_table(plan, X, Ra, Ra_s, reps=20):
     """{stage name: (avg_ms, bytes)} for stages 1-4 of one batched step."""
-    plan.step(X, Ra, Ra_s, nsteps=1, out=torch.empty_like(X))   # fills the workspace
     B = X.shape[0]
+    Ra, Ra_s = plan._params(Ra, B), plan._params(Ra_s, B)          # device-resident once
+    plan.step(X, Ra, Ra_s, nsteps=1, out=torch.empty_like(X))   # fills the workspace
     out = {}
     for st in (1, 2, 3, 4):
         out[STAGES[st]] = (time_stage(plan, X, Ra, Ra_s, st, reps), stage_bytes(plan, B, st))
