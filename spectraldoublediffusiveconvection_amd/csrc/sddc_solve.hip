@@ -44,6 +44,7 @@ namespace sddc {
 
 constexpr int kDepth = 2;    // chain rows of operator data in flight per lane
 constexpr int kDepthE = 2;   // segment exit maps in flight per lane (join)
+constexpr int kDepthM = 4;   // rows in flight in the MFMA kernel (few waves: deep prefetch)
 
 struct ChainArgs {
   const double* G;          // right-hand sides, field f of instance b at b*gstride+goff[f]
@@ -659,8 +660,8 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
     return;
   }
   const double* lb = a.Lm[field] + (size_t)w * NKP * 128 + 2 * lane;
-  double2 blk[kDepth][NKP];
-  double gv[kDepth][4];
+  double2 blk[kDepthM][NKP];
+  double gv[kDepthM][4];
   auto fetch = [&](int k, double2 (&bq)[NKP], double (&g)[4]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
     const double* p = lb + (size_t)(K - 1 - j) * MBLK;
@@ -670,15 +671,15 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
     for (int v = 0; v < 4; ++v) g[v] = gp[(size_t)j * nr + rrv[v]];
   };
 #pragma unroll
-  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  for (int d = 0; d < kDepthM; ++d) fetch(k0 + d, blk[d], gv[d]);
   if (field != 0) {
     // NAB2 (Matrix_Operators.py:1061-1083)
     drain_loads();
     const double dtf = a.dtf[field];
     double bs[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int kb = k0; kb < k1; kb += kDepth) {
+    for (int kb = k0; kb < k1; kb += kDepthM) {
 #pragma unroll
-      for (int d = 0; d < kDepth; ++d) {
+      for (int d = 0; d < kDepthM; ++d) {
         const int k = kb + d;
         const bool live = k < k1;
         const int j = start - 2 * (live ? k : k1 - 1);
@@ -694,7 +695,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
           *((live && okr[v] && valid) ? xp + (size_t)j * nr + rv[v] : sink) = acc[v];
           bs[v] = live ? bs[v] + alpha * acc[v] : bs[v];
         }
-        fetch(k + kDepth, blk[d], gv[d]);
+        fetch(k + kDepthM, blk[d], gv[d]);
       }
     }
 #pragma unroll
@@ -705,7 +706,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
   // state (D2 f = (D2 Linv_j) rhs from the same right-hand side as f)
   const double dta = a.dtf[0];
   const double* mbase = a.DLm + (size_t)w * NKP * 128 + 2 * lane;
-  double2 mfr[kDepth][NKP];
+  double2 mfr[kDepthM][NKP];
   auto fetchm = [&](int k, double2 (&mq)[NKP]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
     const double* p = mbase + (size_t)(K - 1 - j) * MBLK;
@@ -713,7 +714,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
     for (int q = 0; q < NKP; ++q) mq[q] = *reinterpret_cast<const double2*>(p + q * 128);
   };
 #pragma unroll
-  for (int d = 0; d < kDepth; ++d) fetchm(k0 + d, mfr[d]);
+  for (int d = 0; d < kDepthM; ++d) fetchm(k0 + d, mfr[d]);
   double ir4[4], ir2[4], fe[4], bfe[4], dfe[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
@@ -722,9 +723,9 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
     fe[v] = bfe[v] = dfe[v] = 0.0;
   }
   drain_loads();
-  for (int kb = k0; kb < k1; kb += kDepth) {
+  for (int kb = k0; kb < k1; kb += kDepthM) {
 #pragma unroll
-    for (int d = 0; d < kDepth; ++d) {
+    for (int d = 0; d < kDepthM; ++d) {
       const int k = kb + d;
       const bool live = k < k1;
       const int row = start - 2 * (live ? k : k1 - 1);
@@ -749,8 +750,8 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
         fe[v] = live ? fe[v] + acc[v] : fe[v];
         dfe[v] = live ? dfe[v] + accd[v] : dfe[v];
       }
-      fetch(k + kDepth, blk[d], gv[d]);
-      fetchm(k + kDepth, mfr[d]);
+      fetch(k + kDepthM, blk[d], gv[d]);
+      fetchm(k + kDepthM, mfr[d]);
     }
   }
 #pragma unroll
