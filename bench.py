@@ -92,6 +92,9 @@ def sweep_c4(tuning_table, n=256, graph_steps=10, launches=5):
             "ms_per_sweep_step": 1e3 * el / (graph_steps * launches), "states_finite": finite}
 
 
+_WIDE_GAP = {}
+
+
 def serving_run(rate, seconds, tuning_table):
     """Poisson stream of mixed jobs through the serving engine: 60% time
     integrations (500 steps, sizes as in the sweep), 30% Newton solves and 10%
@@ -102,10 +105,12 @@ def serving_run(rate, seconds, tuning_table):
     from spectraldoublediffusiveconvection_amd import device as dev
     from spectraldoublediffusiveconvection_amd import serving
     Kw, Nrw, d = 32, 16, 2.0
-    tsp = dev.Plan.build(Kw, Nrw, d, 0.075, 10.0, 1.0, symmetric=True)
-    x0 = np.random.default_rng(0).random(3 * Kw * (Nrw - 1))
-    X = tsp.step(torch.as_tensor(1e-3 * x0 / np.linalg.norm(x0), device="cuda"), 6780.0, 0.0,
-                 nsteps=26667)[0].cpu().numpy()
+    if "X" not in _WIDE_GAP:   # steady-state start of the Newton and continuation jobs
+        tsp = dev.Plan.build(Kw, Nrw, d, 0.075, 10.0, 1.0, symmetric=True)
+        x0 = np.random.default_rng(0).random(3 * Kw * (Nrw - 1))
+        _WIDE_GAP["X"] = tsp.step(torch.as_tensor(1e-3 * x0 / np.linalg.norm(x0), device="cuda"),
+                                  6780.0, 0.0, nsteps=26667)[0].cpu().numpy()
+    X = _WIDE_GAP["X"]
     case = dict(N_fm=Kw, N_r=Nrw, d=d, Pr=10.0, Tau=1.0, Ra=6780.0, Ra_s=0.0, X=X,
                 symmetric=True, restart=100, dRa=10.0)
     w = WORKLOAD
@@ -201,12 +206,20 @@ def main():
     el64 = time.perf_counter() - t1
     plain = n64 / el64
 
-    # ---- kinetic energy of the mixed method against float64 after 100 steps
+    # ---- mixed method against float64: kinetic energy and Nusselt numbers
+    # every 100 steps over 10^4 steps of one instance
     p32 = dev.Plan(op, precision="mixed")
-    xa = torch.as_tensor(seeded_states(1, n3, seed=5), device="cuda")
-    k64 = p64.diagnostics(p64.step(xa, w["Ra_hi"], w["Ra_s"], nsteps=100))[0, 1].item()
-    k32 = p64.diagnostics(p32.step(xa, w["Ra_hi"], w["Ra_s"], nsteps=100))[0, 1].item()
-    ke_err = abs(k32 / k64 - 1.0)
+    x64 = torch.as_tensor(seeded_states(1, n3, seed=5), device="cuda")
+    x32 = x64.clone()
+    worst = np.zeros(3)
+    for _ in range(100):
+        x64 = p64.step(x64, w["Ra_hi"], w["Ra_s"], nsteps=100)
+        x32 = p32.step(x32, w["Ra_hi"], w["Ra_s"], nsteps=100)
+        d64 = p64.diagnostics(x64)[0, 1:4].cpu().numpy()
+        d32 = p64.diagnostics(x32)[0, 1:4].cpu().numpy()
+        worst = np.maximum(worst, np.abs(d32 / d64 - 1.0))
+    mixed_err = {"steps": 10000, "every": 100, "ke_rel_max": float(worst[0]),
+                 "nu_t_minus_1_rel_max": float(worst[1]), "nu_s_minus_1_rel_max": float(worst[2])}
 
     line = {
         "metric": "aggregate CN/AB2 steps/s at Nr=40,Ntheta=512 vs plain FP64 GPU; p50/p99 job latency",
@@ -227,13 +240,19 @@ def main():
                    "launch_shape": plan.tuning()},
         "plain_fp64_gpu": {"steps_per_s": plain, "ms_per_step": 1e3 * el64 / n64},
         "speedup_vs_plain_fp64": value / plain,
-        "mixed_ke_rel_error_100_steps": ke_err,
+        "mixed_vs_fp64": mixed_err,
         "roofline": roof,
     }
     if not args.no_sweep:
         line["sweep_mixed_sizes"] = sweep_c4(table)
     if not args.no_serving:
         line["serving"] = serving_run(args.serve_rate, args.serve_seconds, table)
+        line["serving_rates"] = []
+        for rate in (25.0, 200.0):
+            st = serving_run(rate, 3.0, table)
+            line["serving_rates"].append({k: st[k] for k in (
+                "rate_jobs_per_s", "submitted", "jobs", "jobs_per_s", "latency_p50_ms",
+                "latency_p99_ms", "instance_steps_per_s")})
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line))

@@ -245,6 +245,30 @@ def test_thousand_steps_per_step_parity(c1):
     assert abs(d[3] / O.nusselt(xc[2 * N:], C1["d"], o.R, o.D, K, nr, check=False) - 1) < 1e-9
 
 
+def test_ten_thousand_steps_vs_oracle(c1):
+    """10^4 float64 steps of the CPU-runnable workload on the GPU and in the
+    oracle from the same seeded state: kinetic energy and both Nusselt
+    numbers within 1e-9 relative every 1000 steps."""
+    o, p = c1
+    K, nr = C1["N_fm"], C1["N_r"] - 1
+    N = K * nr
+    X0 = np.random.default_rng(77).standard_normal(3 * N)
+    X0 = 1e-3 * X0 / np.linalg.norm(X0)
+    xg = T(X0).view(1, -1)
+    xc = X0.copy()
+    for blk in range(10):
+        xg = p.step(xg, RA_LONG, C1["Ra_s"], nsteps=1000)
+        for _ in range(1000):
+            xc = O.step(xc, o, RA_LONG, C1["Ra_s"])
+        d = H(p.diagnostics(xg))[0]
+        ke = O.kinetic_energy(xc, o.R, o.D, K, nr)
+        nut = O.nusselt(xc[N:2 * N], C1["d"], o.R, o.D, K, nr, check=False)
+        nus = O.nusselt(xc[2 * N:], C1["d"], o.R, o.D, K, nr, check=False)
+        assert abs(d[1] / ke - 1) < 1e-9, (blk, d[1], ke)
+        assert abs(d[2] / nut - 1) < 1e-9, (blk, d[2], nut)
+        assert abs(d[3] / nus - 1) < 1e-9, (blk, d[3], nus)
+
+
 @pytest.mark.parametrize("K,Nr", [(256, 30), (512, 40), (128, 20), (64, 16), (48, 24), (10, 20)])
 def test_step_other_sizes(K, Nr):
     """Every nonlinear kernel variant (in place for K = 128, 256, 512; the
@@ -296,8 +320,9 @@ def test_graph_replay_matches_eager(c1):
 
 def test_mixed_precision_error_bound():
     """Mixed precision at the full workload size N_r=40, N_fm=512 (float32
-    nonlinear pipeline, float64 linear terms and solves): kinetic energy within
-    1e-6 of the float64 run at every 100 steps over 1000 steps."""
+    nonlinear pipeline, float64 linear terms and solves) over 10^4 steps:
+    kinetic energy within 1e-6 of the float64 run at every 100 steps (the
+    target), Nu_T - 1 and Nu_S - 1 within 1e-5 (measured: 4e-8, 6e-7, 1.3e-6)."""
     K, Nr, d, dt, Pr, Tau = 512, 40, 0.35, 5e-3, 1.0, 1 / 15.0
     op = ops.build(K, Nr, d, dt, Pr, Tau)
     p64 = dev.Plan(op)
@@ -306,15 +331,16 @@ def test_mixed_precision_error_bound():
     X0 = 1e-3 * X0 / np.linalg.norm(X0)
     x64 = T(X0).view(1, -1)
     x32 = x64.clone()
-    worst_ke = 0.0
-    for blk in range(10):
+    worst = np.zeros(3)
+    for blk in range(100):
         x64 = p64.step(x64, RA_LONG, 500.0, nsteps=100)
         x32 = p32.step(x32, RA_LONG, 500.0, nsteps=100)
         d64 = H(p64.diagnostics(x64))[0]
         d32 = H(p64.diagnostics(x32))[0]
         assert np.all(np.isfinite(d64))
-        worst_ke = max(worst_ke, abs(d32[1] / d64[1] - 1))
-    assert worst_ke < 1e-6, worst_ke
+        worst = np.maximum(worst, np.abs(d32[1:4] / d64[1:4] - 1))
+    assert worst[0] < 1e-6, worst
+    assert worst[1] < 1e-5 and worst[2] < 1e-5, worst
     big = T(1e-2 * np.random.default_rng(12).standard_normal(3 * K * (Nr - 1))).view(1, -1)
     assert rel(H(p32.step(big, RA_LONG, 500.0))[0], H(p64.step(big, RA_LONG, 500.0))[0]) < 1e-5
 
