@@ -12,6 +12,7 @@
 // with K = N_fm modes, nr = N_r - 1 interior Chebyshev points, and real_t the
 // nonlinear-pipeline precision (float64 plain, float32 mixed).
 #pragma once
+#include <atomic>
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -148,7 +149,7 @@ int launch_diag_final(const Plan& p, int B, const double* X, const double* rowva
                       double* out, hipStream_t s);
 int nonlinear_lds_bytes(const Plan& p, int precision, int dfx);
 // Raise a kernel's dynamic-LDS limit to `bytes` (once per size increase).
-int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current);
+int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& current);
 
 }  // namespace sddc
 

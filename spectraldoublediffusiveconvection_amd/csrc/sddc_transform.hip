@@ -1,6 +1,8 @@
 // FFT-based kernels: generic transforms (Transforms.py), the fused nonlinear
 // advection kernel (NLIN_FX / NLIN_DFX, Matrix_Operators.py:743-898) and the
 // radial rows of the kinetic energy (Main.py:71-134).
+#include <atomic>
+
 #include "sddc_fft.h"
 
 namespace sddc {
@@ -89,7 +91,7 @@ int launch_transform(const TableSet& t, int kind, int rows, int len_in, int n_ou
   if (rows <= 0) return SDDC_OK;
   const size_t lds = 2 * (size_t)t.L * sizeof(double2);
   if (lds > 160 * 1024) return fail(SDDC_ERR_UNSUPPORTED, "transform length too large for LDS");
-  static size_t cur = 0;
+  static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_transform, lds, cur)) return rc;
   hipLaunchKernelGGL(k_transform, dim3((rows + 1) / 2), dim3(256), lds, s, t.shape, t.tw64,
                      t.rot64, kind, rows, len_in, n_out, in, out);
@@ -935,14 +937,19 @@ static int nl_inplace_nc(const Plan& p) {
   return ip_shape_ok(p.tabM.shape) ? nc : 0;
 }
 
-int ensure_dynamic_lds(const void* fn, size_t bytes, size_t& current) {
-  if (bytes <= 64 * 1024 || bytes <= current) return SDDC_OK;
+// Raises a kernel's dynamic-LDS limit to `bytes` once.  `current` is the
+// limit already set for that kernel; calls may come from several host
+// threads (the attribute is idempotent, the record is atomic).
+int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& current) {
+  if (bytes <= 64 * 1024 || bytes <= current.load(std::memory_order_acquire)) return SDDC_OK;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();   // do not leave the error for the next launch
     return fail(SDDC_ERR_UNSUPPORTED, "dynamic LDS request above the device limit");
   }
-  current = bytes;
+  size_t prev = current.load(std::memory_order_relaxed);
+  while (prev < bytes && !current.compare_exchange_weak(prev, bytes, std::memory_order_acq_rel)) {
+  }
   return SDDC_OK;
 }
 
@@ -957,7 +964,7 @@ template <typename R, int NC>
 static int launch_nl_ip(const Plan& p, int B, const void* slabs, void* F, hipStream_t s,
                         const cplx<R>* tw, const cplx<R>* rot) {
   const size_t lds = 8 * (size_t)(64 * NC) * sizeof(cplx<R>);
-  static size_t cur = 0;
+  static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_ip<R, NC>, lds, cur)) return rc;
   hipLaunchKernelGGL((k_nonlinear_ip<R, NC>), dim3(p.nr, B), dim3(320), lds, s, p.tabM.shape, tw,
                      rot, p.nr, (const R*)slabs, (R*)F);
@@ -968,7 +975,7 @@ template <typename R, int NC>
 static int launch_nl_dfx_ip(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
                             hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
   const size_t lds = 13 * (size_t)(64 * NC) * sizeof(cplx<R>);
-  static size_t cur = 0;
+  static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_dfx_ip<R, NC>, lds, cur)) return rc;
   hipLaunchKernelGGL((k_nonlinear_dfx_ip<R, NC>), dim3(p.nr, B), dim3(640), lds, s, p.tabM.shape,
                      tw, rot, p.nr, (const R*)slabs, (const R*)slabs2, (R*)F);
@@ -988,7 +995,7 @@ static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slab
     }
     if (lds + 64 > 160 * 1024)
       return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
-    static size_t curw = 0;
+    static std::atomic<size_t> curw{0};
     if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_w<R>, lds, curw)) return rc;
     hipLaunchKernelGGL((k_nonlinear_w<R>), dim3(p.nr, B), dim3(320), lds, s, p.tabM.shape, tw,
                        rot, p.K, p.nr, (const R*)slabs, (R*)F);
@@ -1002,7 +1009,7 @@ static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slab
   }
   if (lds + 16 * 18 * sizeof(int) > 160 * 1024)
     return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
-  static size_t cur = 0;
+  static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_nonlinear<R, DFX>, lds, cur)) return rc;
   hipLaunchKernelGGL((k_nonlinear<R, DFX>), dim3(p.nr, B), dim3(1024), lds, s, p.tabM.shape, tw,
                      rot, p.K, p.nr, (const R*)slabs, (const R*)slabs2, (R*)F);
@@ -1091,7 +1098,7 @@ __global__ __launch_bounds__(256) void k_ke_rows(FftShape sh, const double2* tw_
 int launch_ke_rows(const Plan& p, int B, const void* slabs64, double* rowval, hipStream_t s) {
   const size_t lds = 2 * (size_t)p.tabKE.L * sizeof(double2);
   if (lds > 160 * 1024) return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS energy kernel");
-  static size_t cur = 0;
+  static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_ke_rows, lds, cur)) return rc;
   hipLaunchKernelGGL(k_ke_rows, dim3(p.nr, B), dim3(256), lds, s, p.tabKE.shape, p.tabKE.tw64,
                      p.tabKE.rot64, p.K, p.nr, (const double*)slabs64, p.vecs, p.NRP, rowval);
