@@ -71,7 +71,8 @@ typedef struct sddc_plan sddc_plan;
  *   chain_segment    rows per segment of the parity chains of the solves
  *                    (>= rows per chain: one segment, no join)
  *   solve_instances  instances per solve workgroup: 1, 2 or 4 (quad-lane
- *                    kernels), 16 (MFMA kernel; best for large batches)
+ *                    kernels), 3 (quad-lane, 2 per A4 workgroup and 4 per
+ *                    NAB2 workgroup), 16 (MFMA kernel)
  *   sub_batches      graphs only: the batch is split into this many parts
  *                    (1-8), each stepped on its own forked stream inside the
  *                    graph, so that the latency-bound solves of one part

@@ -354,6 +354,60 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs 
     local_nab2<NRS, NB>(a, field, start, k0, k1, in, xs, exit_seg);
 }
 
+// k_chain_local with two widths in one launch (solve_instances = 3): the A4
+// chains, with two products per row, take 2 instances per workgroup and the
+// NAB2 chains 4.  The launch has ceil(B/2) groups; NAB2 workgroups past
+// ceil(B/4) return at once.  Register use is the larger of the two paths,
+// well below that of 4 instances on the A4 path.
+template <int NRS>
+__global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(ChainArgs a) {
+  using S = CShape<NRS>;
+  __shared__ __align__(16) double xs[2][4][NRS + 64];
+  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y;
+  if (!((a.fields_mask >> field) & 1)) return;
+  if (field != 0 && (int)blockIdx.z * 4 >= a.B) return;
+  const int start = a.K - 1 - chain;
+  const int n = start / 2 + 1;
+  const int k0 = seg * a.m;
+  if (k0 >= n) return;
+  const int k1 = k0 + a.m < n ? k0 + a.m : n;
+  double* exit_seg = a.EX + (size_t)seg * 2 * S::NRP;
+  const bool sym_zero = a.symmetric && (field == 0 ? chain == 1 : chain == 0);
+  if (field == 0) {
+    const Inst<2> in = instances<2>(a, field, chain);
+    if (sym_zero) {   // Eq_SYM: this chain is zero (Matrix_Operators.py:1166)
+      const int lane = threadIdx.x & 63;
+      const int r = 16 * (threadIdx.x >> 6) + (lane >> 2);
+      if ((lane & 3) == 0 && r < a.nr)
+        for (int k = k0; k < k1; ++k) {
+          const size_t o = (size_t)(start - 2 * k) * a.nr + r;
+#pragma unroll
+          for (int t = 0; t < 2; ++t) {
+            in.x[t][o] = 0.0;
+            in.p[t][o] = 0.0;
+          }
+        }
+      return;
+    }
+    local_a4<NRS, 2>(a, start, k0, k1, in, reinterpret_cast<double (*)[2][NRS + 64]>(xs),
+                     exit_seg);
+    return;
+  }
+  const Inst<4> in = instances<4>(a, field, chain);
+  if (sym_zero) {   // Eq_SYM: this chain is zero (Matrix_Operators.py:1054)
+    const int lane = threadIdx.x & 63;
+    const int r = 16 * (threadIdx.x >> 6) + (lane >> 2);
+    if ((lane & 3) == 0 && r < a.nr)
+      for (int k = k0; k < k1; ++k) {
+        const size_t o = (size_t)(start - 2 * k) * a.nr + r;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) in.x[t][o] = 0.0;
+      }
+    return;
+  }
+  local_nab2<NRS, 4>(a, field, start, k0, k1, in, xs, exit_seg);
+}
+
 // ---------------------------------------------------------------------------
 // k_chain_join: entry state of a segment, then the row corrections.
 // ---------------------------------------------------------------------------
@@ -789,10 +843,24 @@ static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
 }
 
 template <int NRS>
+static int launch_chain_mixed_t(const ChainArgs& a, int B, hipStream_t s) {
+  using S = CShape<NRS>;
+  const dim3 grid(6, a.nseg, (B + 1) / 2), block(64 * S::WR);
+  hipLaunchKernelGGL((k_chain_local_mixed<NRS>), grid, block, 0, s, a);
+  if (int rc = hip_check(hipGetLastError(), "k_chain_local_mixed")) return rc;
+  if (a.nseg > 1) {
+    hipLaunchKernelGGL((k_chain_join<NRS, 2>), grid, block, 0, s, a);
+    if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
+  }
+  return SDDC_OK;
+}
+
+template <int NRS>
 static int dispatch_nb(const ChainArgs& a, int NB, int B, hipStream_t s) {
   switch (NB) {
     case 1: return launch_chain_t<NRS, 1>(a, B, s);
     case 2: return launch_chain_t<NRS, 2>(a, B, s);
+    case 3: return launch_chain_mixed_t<NRS>(a, B, s);
     case 4: return launch_chain_t<NRS, 4>(a, B, s);
     default: return fail(SDDC_ERR_INVALID, "unsupported instances per workgroup");
   }

@@ -151,17 +151,19 @@ def test_solve_segmentations(K, Nr, max_batch, sym):
 @pytest.mark.parametrize("K,Nr", [(64, 20), (512, 40), (10, 9), (32, 65)])
 @pytest.mark.parametrize("segment", [0, 8])
 @pytest.mark.parametrize("sym", [False, True])
-def test_solve_mfma(K, Nr, segment, sym):
-    """The 16-instance MFMA chain solve (solve_instances = 16) against the
-    oracle's back substitution, for a ragged batch of 21 (one full and one
-    partial group of 16), unsegmented and in segments of 8 rows, then three
-    steps against the oracle."""
+@pytest.mark.parametrize("width", [16, 3])
+def test_solve_mfma(K, Nr, segment, sym, width):
+    """The 16-instance MFMA chain solve (solve_instances = 16) and the
+    mixed-width quad solve (3: 2 instances per A4 workgroup, 4 per NAB2
+    workgroup) against the oracle's back substitution, for a ragged batch of
+    21 (partial workgroups), unsegmented and in segments of 8 rows, then
+    three steps against the oracle."""
     d, dt, Pr, Tau = 0.35, 5e-3, 1.0, 1 / 15.0
     o = O.build_operators(K, Nr, d, dt, Pr, Tau)
     B = 21
     p = dev.Plan(ops.build(K, Nr, d, dt, Pr, Tau), symmetric=sym, max_batch=B,
-                 tuning={"chain_segment": segment, "solve_instances": 16})
-    assert p.tuning()["solve_instances"] == 16
+                 tuning={"chain_segment": segment, "solve_instances": width})
+    assert p.tuning()["solve_instances"] == width
     nr = Nr - 1
     rng = np.random.default_rng(K + segment)
     g = rng.standard_normal((B, K * nr)) * np.logspace(0, -3, K).repeat(nr)[None, :]
