@@ -27,6 +27,28 @@ constexpr int kMaxRadices = 24;
 constexpr int kTile = 16;          // modes per workgroup in the spectral stages
 constexpr int kMaxNr = 64;         // largest supported interior point count
 
+// Waits for every memory operation in flight.  Issued once after a loop's
+// prologue (its first fetches and the loop-invariant operands): without it
+// the compiler's wait-count analysis merges the prologue's issue order into
+// the loop header and waits, on every trip, for loads issued for a later
+// trip, which serialises the loop on memory latency.
+__device__ __forceinline__ void drain_loads() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);   // vmcnt 0, expcnt 0, lgkmcnt 0
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// Workgroup barrier for data exchanged through LDS: the wavefront's own LDS
+// accesses are complete (lgkmcnt 0) before s_barrier.  No vmcnt wait, so
+// global loads in flight stay in flight.  Signal fences keep the compiler
+// from moving LDS accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0xC07F);   // vmcnt 63 (none), expcnt 7 (none), lgkmcnt 0
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 template <typename R>
 struct cplx {
   R x, y;

@@ -92,28 +92,6 @@ __device__ __forceinline__ double quad_sum(double v) {
   return v;
 }
 
-// Workgroup barrier for the LDS broadcast: the wavefront's own LDS writes are
-// complete (lgkmcnt(0)) before s_barrier.  No vmcnt wait, so loads in flight
-// stay in flight.  Signal fences keep the compiler from moving LDS accesses
-// across it.
-// Waits for every load in flight.  Issued once after a loop's prologue
-// (the first kDepth fetches and the loop-invariant operands): without it the
-// compiler's wait-count analysis merges the prologue's issue order into the
-// loop header and waits, on every trip, for loads issued one row earlier,
-// which serialises the chain on memory latency.
-__device__ __forceinline__ void drain_loads() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0);   // vmcnt 0, expcnt 0, lgkmcnt 0
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-
-__device__ __forceinline__ void lds_barrier() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0xC07F);   // vmcnt 63 (none), expcnt 7 (none), lgkmcnt 0
-  __builtin_amdgcn_s_barrier();
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-
 template <int NQ>
 __device__ __forceinline__ void load_blk(double2 (&a)[NQ], const double* p) {
 #pragma unroll

@@ -69,8 +69,13 @@ int launch_suffix(const Plan& p, int B, const double* X, double* P, hipStream_t 
 }
 
 // ---------------------------------------------------------------------------
-// Spectral stage: one workgroup per (tile of 16 modes, instance).
+// Spectral stage: one workgroup per (tile of 16 modes, run of nb instances).
+// The workgroup loads its Chebyshev operator tiles once and walks its
+// instances; the state tiles of instance u+1 are loaded into registers while
+// instance u is computed.
 // ---------------------------------------------------------------------------
+constexpr int kMaxSpectralInstances = 4;   // instances per spectral workgroup, at most
+
 struct SpectralArgs {
   const double* X;
   const double* P;
@@ -83,18 +88,49 @@ struct SpectralArgs {
   void* slabs;         // [b][7][nr][K] (real_t) or null
   double* theta;       // [b][4][K][nr] or null
   double* slab9;       // [b][9][nr][K] or null
-  int K, nr;
+  int K, nr, B, nb;    // nb: instances per workgroup
   double dt, Pr;
 };
+
+// Contraction units of the spectral stage's MFMA phase, (contraction << 3) |
+// row tile, for wavefront w (of five) and its j-th unit; -1: none.  float64
+// plans: wavefront c runs contraction c.  Mixed plans: the float64 units
+// (contraction 2) go one per wavefront, first in its list; the float units
+// fill the wavefronts up to about equal matrix-core time (a float64 unit
+// counts twice).
+template <bool kMixed, int NT>
+__host__ __device__ constexpr int spectral_units() {
+  return !kMixed ? NT : (NT == 1 ? 1 : (NT == 2 ? 3 : (NT == 3 ? 3 : 4)));
+}
+
+template <bool kMixed, int NT>
+__device__ __forceinline__ int spectral_unit(int w, int j) {
+  if (!kMixed) return j < NT ? (w << 3) | j : -1;
+  constexpr int U = spectral_units<kMixed, NT>();
+  // rows: wavefronts; entries: (c << 3) | it, -1 = none
+  constexpr int t1[5][1] = {{0}, {8}, {16}, {24}, {32}};
+  constexpr int t2[5][3] = {{16, -1, -1}, {17, -1, -1}, {0, 1, 8}, {9, 24, 25}, {32, 33, -1}};
+  constexpr int t3[5][3] = {{16, 0, 1}, {17, 2, 8}, {18, 9, 10}, {24, 25, 26}, {32, 33, 34}};
+  constexpr int t4[5][4] = {{16, 0, 1, 2}, {17, 3, 8, 9}, {18, 10, 11, 24}, {19, 25, 26, 27},
+                            {32, 33, 34, 35}};
+  if (j >= U) return -1;
+  if constexpr (NT == 1) return t1[w][j];
+  else if constexpr (NT == 2) return t2[w][j];
+  else if constexpr (NT == 3) return t3[w][j];
+  else return t4[w][j];
+}
 
 template <typename R, int NRP>
 __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
   constexpr int NT = NRP / 16;
   constexpr int NK = NRP / 4;
-  __shared__ double sPsi[kTile + 2][NRP];   // modes k0-1 .. k0+16
-  __shared__ double sP[kTile + 1][NRP];     // modes k0 .. k0+16
-  __shared__ double sT[kTile + 1][NRP];
-  __shared__ double sS[kTile + 1][NRP];
+  // rows padded to an odd number of doubles: the MFMA operand reads take one
+  // column of 16 rows, which then fall in different LDS banks
+  constexpr int LD = NRP + 1;
+  __shared__ double sPsi[kTile + 2][LD];   // modes k0-1 .. k0+16
+  __shared__ double sP[kTile + 1][LD];     // modes k0 .. k0+16
+  __shared__ double sT[kTile + 1][LD];
+  __shared__ double sS[kTile + 1][LD];
   // contraction results [mode][i]: Dr psi, Dr T, Dr S in the precision they
   // are computed in (float in mixed precision: 3 workgroups fit per CU);
   // D2r2 psi and D2s psi in float64 (overwritten by omega and J_theta psi)
@@ -102,54 +138,45 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
   __shared__ double sCd[2][kTile][NRP + 1];
   constexpr int kVecs = V_GR + 1;           // IR2, IR4, DT0, RSQ, GR
   __shared__ double sV[kVecs * NRP];        // radial vectors
+  __shared__ double sRa[2][kMaxSpectralInstances];   // Ra_T, Ra_S of the instances
   const int K = a.K, nr = a.nr;
-  const int b = blockIdx.y;
+  const int bfirst = blockIdx.y * a.nb;
+  const int nb = min(a.nb, a.B - bfirst);
   const int k0 = blockIdx.x * kTile;
   const size_t N = (size_t)K * nr;
-  const double* Xb = a.X + (size_t)b * 3 * N;
-  const double* Pb = a.P + (size_t)b * N;
 
-  // Wavefront c runs contraction c over all row tiles:
+  // Contractions (nr x nr operator times a 16-mode tile), each split into
+  // NT row tiles ("units"):
   //  0 Dr psi, 1 D2r2 psi, 2 D2s psi (float64 always: linear term), 3 Dr T, 4 Dr S.
-  // Its operator tiles are loaded first, all in flight together with the
-  // state tiles below (every global load of the kernel is issued before any
-  // is waited for).
-  const int c = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // Wavefront w runs the units spectral_unit<mixed, NT>(w, j), j < kUnits.
+  // In mixed precision the float64 units, which take twice as long on the
+  // matrix cores, are spread one per wavefront; unit 0 of a wavefront is the
+  // only one that may be float64, so the other units' tiles are held as
+  // floats.
+  constexpr bool kMixed = std::is_same<R, float>::value;
+  constexpr int kUnits = spectral_units<kMixed, NT>();
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int col = lane & 15, kq = lane >> 4;
-  const bool f64 = (c == 2) || std::is_same<R, double>::value;
-  const int slot = (c == 1) ? 1 : (c == 2 ? 2 : 0);
-  // one register array for both precisions (a float tile is held exactly
-  // in doubles), so the two paths do not each reserve registers
-  double a64[NT][NK];
-  if (f64) {
-    const double* At = a.A64 + (size_t)slot * NT * NK * 64 + lane;
+  double a0[NK];                                    // unit 0 (float64 or float)
+  float af[kUnits > 1 ? kUnits - 1 : 1][NK];        // units 1.. (mixed: float)
+  double ad[kMixed ? 1 : (kUnits > 1 ? kUnits - 1 : 1)][NK];   // units 1.. (float64 plan)
 #pragma unroll
-    for (int it = 0; it < NT; ++it)
+  for (int j = 0; j < kUnits; ++j) {
+    const int code = spectral_unit<kMixed, NT>(w, j);
+    if (code < 0) continue;
+    const int cc = code >> 3, it = code & 7;
+    const int slot = (cc == 1) ? 1 : (cc == 2 ? 2 : 0);   // Dr, D2r2, D2s
+    const bool f64 = !kMixed || cc == 2;
+    const size_t base = (size_t)slot * NT * NK * 64 + (size_t)it * NK * 64 + lane;
 #pragma unroll
-      for (int kk = 0; kk < NK; ++kk) a64[it][kk] = At[(it * NK + kk) * 64];
-  } else {
-    const float* At = a.A32 + (size_t)slot * NT * NK * 64 + lane;
-#pragma unroll
-    for (int it = 0; it < NT; ++it)
-#pragma unroll
-      for (int kk = 0; kk < NK; ++kk) a64[it][kk] = (double)At[(it * NK + kk) * 64];
-  }
-
-  constexpr int kLoadIters = ((kTile + 2) * NRP + 319) / 320;
-#pragma unroll
-  for (int it = 0; it < kLoadIters; ++it) {
-    const int idx = threadIdx.x + 320 * it;
-    if (idx < (kTile + 2) * NRP) {
-      const int t = idx / NRP, i = idx - t * NRP;
-      const int k = k0 - 1 + t;
-      const bool ok = (k >= 0 && k < K && i < nr);
-      const size_t o = ok ? (size_t)k * nr + i : 0;
-      sPsi[t][i] = ok ? Xb[o] : 0.0;
-      if (t >= 1) {
-        sP[t - 1][i] = ok ? Pb[o] : 0.0;
-        sT[t - 1][i] = ok ? Xb[N + o] : 0.0;
-        sS[t - 1][i] = ok ? Xb[2 * N + o] : 0.0;
-      }
+    for (int kk = 0; kk < NK; ++kk) {
+      const double v = f64 ? a.A64[base + kk * 64] : (double)a.A32[base + kk * 64];
+      if (j == 0)
+        a0[kk] = v;
+      else if (kMixed)
+        af[j - 1][kk] = (float)v;
+      else
+        ad[kMixed ? 0 : j - 1][kk] = v;
     }
   }
   constexpr int kVecIters = (kVecs * NRP + 319) / 320;
@@ -158,6 +185,38 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
     const int e = threadIdx.x + 320 * it;
     if (e < kVecs * NRP) sV[e] = a.vecs[e];
   }
+  // the parameters are staged with the prologue: a global load inside the
+  // loop would be waited for together with the next instance's prefetch
+  if (threadIdx.x < 2 * kMaxSpectralInstances) {
+    const int u = threadIdx.x % kMaxSpectralInstances, q = threadIdx.x / kMaxSpectralInstances;
+    const double* src = q == 0 ? a.Ra : a.Ras;
+    sRa[q][u] = (src && u < nb) ? src[bfirst + u] : 0.0;
+  }
+
+  // state tiles of one instance, held in registers between their load and
+  // their store to LDS: psi (modes k0-1 ..), P, T, S (modes k0 ..)
+  constexpr int kLoadIters = ((kTile + 2) * NRP + 319) / 320;
+  constexpr int kOutIters = (kTile * NRP + 319) / 320;   // outputs per thread, at most
+  double st[kLoadIters][4];
+  auto fetch = [&](int b, int tx) {
+    const double* Xb = a.X + (size_t)b * 3 * N;
+    const double* Pb = a.P + (size_t)b * N;
+#pragma unroll
+    for (int it = 0; it < kLoadIters; ++it) {
+      const int idx = tx + 320 * it;
+      const int t = idx / NRP, i = idx - t * NRP;
+      const int k = k0 - 1 + t;
+      const bool ok = idx < (kTile + 2) * NRP && k >= 0 && k < K && i < nr;
+      // every lane loads (entry 0 where out of range; the store to LDS
+      // writes zero there): no branch or select waits for these loads,
+      // which stay in flight until their values are stored
+      const size_t o = ok ? (size_t)k * nr + i : 0;
+      st[it][0] = Xb[o];
+      st[it][1] = Pb[o];
+      st[it][2] = Xb[N + o];
+      st[it][3] = Xb[2 * N + o];
+    }
+  };
   auto put = [&](int cc, int rl, int i, double v) {
     if (cc == 1)
       sCd[0][rl][i] = v;
@@ -166,142 +225,199 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
     else
       sCr[cc == 0 ? 0 : cc - 2][rl][i] = (R)v;   // exact: a value of type R
   };
-  __syncthreads();
-
-  // (nr x nr) @ (nr x 16 modes) on MFMA.  Both shapes take A[i][k] from lane
-  // i + 16k and B[k][j] from lane j + 16k; their output layouts differ.
-  {
-    const double* src = (c <= 2) ? &sPsi[1][0] : (c == 3 ? &sT[0][0] : &sS[0][0]);
-    const double* srow = src + col * NRP + kq;
-#pragma unroll
-    for (int it = 0; it < NT; ++it) {
-      if (f64) {
-        v4d acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kk = 0; kk < NK; ++kk)
-          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a64[it][kk], srow[4 * kk], acc, 0, 0, 0);
-        // float64 16x16x4 output: lane l, register v holds row l/16 + 4v
-#pragma unroll
-        for (int v = 0; v < 4; ++v) put(c, col, 16 * it + kq + 4 * v, acc[v]);
-      } else {
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kk = 0; kk < NK; ++kk)
-          acc = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a64[it][kk], (float)srow[4 * kk], acc, 0,
-                                                     0, 0);
-        // float32 16x16x4 output: lane l, register v holds row 4*(l/16) + v
-#pragma unroll
-        for (int v = 0; v < 4; ++v) put(c, col, 16 * it + 4 * kq + v, (double)acc[v]);
-      }
-    }
-  }
-  __syncthreads();
-
   const double* vIR2 = sV + V_IR2 * NRP;
   const double* vIR4 = sV + V_IR4 * NRP;
   const double* vDT0 = sV + V_DT0 * NRP;
   const double* vRSQ = sV + V_RSQ * NRP;
   const double* vGR = sV + V_GR * NRP;
-  const double Ra = a.Ra ? a.Ra[b] : 0.0;
-  const double Ras = a.Ras ? a.Ras[b] : 0.0;
   const double dtpr = a.dt * a.Pr;
-  double* Lb = a.L ? a.L + (size_t)b * 3 * N : nullptr;
-  double* Th = a.theta ? a.theta + (size_t)b * 4 * N : nullptr;
 
-  for (int idx = threadIdx.x; idx < kTile * nr; idx += blockDim.x) {
-    const int rl = idx / nr, i = idx - rl * nr;
-    const int r = k0 + rl;
-    if (r >= K) continue;
-    const double psi = sPsi[rl + 1][i];
-    const double j = r + 1.0;                        // sine wavenumber of row r
-    const double t = (j + 1.0) * psi + 2.0 * sP[rl][i];
-    // A2_SINE_R2 (Matrix_Operators.py:510) and A2_SINE (:230)
-    const double Om = sCd[0][rl][i] - j * (vIR4[i] * t);
-    const double A2 = sCd[1][rl][i] - (j * vIR2[i]) * t;
-    // J_theta_RT (Matrix_Operators.py:449-455): b = inclusive same-parity sum from row r+1
-    const double Q = sPsi[rl + 2][i] + sP[rl + 1][i];
-    const double JP = (r == 0) ? 1.0 * Q : (r + 1.0) * sPsi[rl][i] + 2.0 * Q;
-    const size_t o = (size_t)r * nr + i;
-    const double psiT0 = vDT0[i] * JP;               // DT0_theta (Matrix_Operators.py:170-172)
-    if (Lb) {
-      // Step_Python right-hand sides (Main.py:271, 276, 280)
-      double grv = 0.0;
-      if (r <= K - 2) {
-        const double v = Ra * sT[rl + 1][i] - Ras * sS[rl + 1][i];
-        grv = ((double)(-(r + 1)) * vGR[i]) * v;     // kGR_RT row r <- cosine row r+1
+  fetch(bfirst, threadIdx.x);
+  // the operator tiles are complete before the loop: the compiler's wait
+  // counts inside it then track only the loads of the next instance
+  drain_loads();
+  for (int u = 0; u < nb; ++u) {
+    const int b = bfirst + u;
+    // the thread index, opaque to the optimiser: index arithmetic is redone
+    // per instance instead of being held in registers across the loop
+    int tx = threadIdx.x;
+    asm volatile("" : "+v"(tx));
+    if (u > 0) lds_barrier();   // the previous instance is done with every LDS array
+#pragma unroll
+    for (int it = 0; it < kLoadIters; ++it) {
+      const int idx = tx + 320 * it;
+      if (idx < (kTile + 2) * NRP) {
+        const int t = idx / NRP, i = idx - t * NRP;
+        const int k = k0 - 1 + t;
+        const bool ok = k >= 0 && k < K && i < nr;
+        sPsi[t][i] = ok ? st[it][0] : 0.0;
+        if (t >= 1) {
+          sP[t - 1][i] = ok ? st[it][1] : 0.0;
+          sT[t - 1][i] = ok ? st[it][2] : 0.0;
+          sS[t - 1][i] = ok ? st[it][3] : 0.0;
+        }
       }
-      Lb[o] = A2 + dtpr * grv;
-      Lb[N + o] = vRSQ[i] * sT[rl][i] - a.dt * psiT0;
-      Lb[2 * N + o] = vRSQ[i] * sS[rl][i] - a.dt * psiT0;
     }
-    if (Th) {
-      Th[o] = JP;
-      Th[N + o] = psiT0;
-      Th[2 * N + o] = A2;
-      Th[3 * N + o] = Om;
-    }
-    sCd[0][rl][i] = Om;   // same thread read these entries above
-    sCd[1][rl][i] = JP;
-  }
-  __syncthreads();
+    if (u + 1 < nb) fetch(b + 1, tx);
+    lds_barrier();
 
-  if (a.slabs) {
-    const size_t nK = (size_t)nr * K;
-    R* S7 = reinterpret_cast<R*>(a.slabs) + (size_t)b * 7 * nK;
-    for (int idx = threadIdx.x; idx < nr * kTile; idx += blockDim.x) {
-      const int i = idx / kTile, rl = idx - i * kTile;
-      const int r = k0 + rl;
-      if (r >= K) continue;
-      const size_t row = (size_t)i * K;
-      S7[row + r] = (R)sCd[1][rl][i];
-      S7[3 * nK + row + r] = (R)sCr[1][rl][i];
-      S7[4 * nK + row + r] = (R)sCr[2][rl][i];
-      S7[5 * nK + row + r] = (R)sT[rl][i];
-      S7[6 * nK + row + r] = (R)sS[rl][i];
-      if (r + 1 < K) {   // sine rows move to the sinusoid index r+1 (Matrix_Operators.py:734-737)
-        S7[nK + row + r + 1] = (R)sCr[0][rl][i];
-        S7[2 * nK + row + r + 1] = (R)sCd[0][rl][i];
-      }
-      if (r == 0) {
-        S7[nK + row] = (R)0;
-        S7[2 * nK + row] = (R)0;
+    // (nr x nr) @ (nr x 16 modes) on MFMA.  Both shapes take A[i][k] from lane
+    // i + 16k and B[k][j] from lane j + 16k; their output layouts differ.
+    {
+      // the tile registers are opaque to the optimiser here, so that it does
+      // not keep converted copies of them live across the instance loop
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) asm volatile("" : "+v"(a0[kk]));
+#pragma unroll
+      for (int j = 0; j < kUnits; ++j) {
+        const int code = spectral_unit<kMixed, NT>(w, j);
+        if (code < 0) continue;
+        const int cc = code >> 3, it = code & 7;
+        const double* src = (cc <= 2) ? &sPsi[1][0] : (cc == 3 ? &sT[0][0] : &sS[0][0]);
+        const double* srow = src + col * LD + kq;
+        if (!kMixed || cc == 2) {
+          v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int kk = 0; kk < NK; ++kk) {
+            const double av = (j == 0) ? a0[kk] : ad[kMixed ? 0 : j - 1][kk];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, srow[4 * kk], acc, 0, 0, 0);
+          }
+          // float64 16x16x4 output: lane l, register v holds row l/16 + 4v
+#pragma unroll
+          for (int v = 0; v < 4; ++v) put(cc, col, 16 * it + kq + 4 * v, acc[v]);
+        } else {
+          v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kk = 0; kk < NK; ++kk) {
+            const float av = (j == 0) ? (float)a0[kk] : af[j > 0 ? j - 1 : 0][kk];
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, (float)srow[4 * kk], acc, 0, 0, 0);
+          }
+          // float32 16x16x4 output: lane l, register v holds row 4*(l/16) + v
+#pragma unroll
+          for (int v = 0; v < 4; ++v) put(cc, col, 16 * it + 4 * kq + v, (double)acc[v]);
+        }
       }
     }
-  }
-  if (a.slab9) {
-    const size_t nK = (size_t)nr * K;
-    double* S9 = a.slab9 + (size_t)b * 9 * nK;
-    for (int idx = threadIdx.x; idx < nr * kTile; idx += blockDim.x) {
-      const int i = idx / kTile, rl = idx - i * kTile;
+    lds_barrier();
+
+    const double Ra = sRa[0][u];
+    const double Ras = sRa[1][u];
+    double* Lb = a.L ? a.L + (size_t)b * 3 * N : nullptr;
+    double* Th = a.theta ? a.theta + (size_t)b * 4 * N : nullptr;
+#pragma unroll
+    for (int it = 0; it < kOutIters; ++it) {
+      const int idx = tx + 320 * it;
+      const int rl = idx / nr, i = idx - rl * nr;
       const int r = k0 + rl;
-      if (r >= K) continue;
-      const size_t row = (size_t)i * K;
-      // order: JT, kDpsi, komega, DT, DC, omega, Dpsi, kT, kC (Matrix_Operators.py:740)
-      S9[row + r] = sCd[1][rl][i];
-      S9[3 * nK + row + r] = sCr[1][rl][i];
-      S9[4 * nK + row + r] = sCr[2][rl][i];
-      S9[7 * nK + row + r] = (double)(-r) * sT[rl][i];
-      S9[8 * nK + row + r] = (double)(-r) * sS[rl][i];
-      if (r + 1 < K) {
-        const double ks = r + 1.0;
-        S9[1 * nK + row + r + 1] = ks * sCr[0][rl][i];
-        S9[2 * nK + row + r + 1] = ks * sCd[0][rl][i];
-        S9[5 * nK + row + r + 1] = sCd[0][rl][i];
-        S9[6 * nK + row + r + 1] = sCr[0][rl][i];
+      if (idx >= kTile * nr || r >= K) continue;
+      const double psi = sPsi[rl + 1][i];
+      const double j = r + 1.0;                        // sine wavenumber of row r
+      const double t = (j + 1.0) * psi + 2.0 * sP[rl][i];
+      // A2_SINE_R2 (Matrix_Operators.py:510) and A2_SINE (:230)
+      const double Om = sCd[0][rl][i] - j * (vIR4[i] * t);
+      const double A2 = sCd[1][rl][i] - (j * vIR2[i]) * t;
+      // J_theta_RT (Matrix_Operators.py:449-455): b = inclusive same-parity sum from row r+1
+      const double Q = sPsi[rl + 2][i] + sP[rl + 1][i];
+      const double JP = (r == 0) ? 1.0 * Q : (r + 1.0) * sPsi[rl][i] + 2.0 * Q;
+      const size_t o = (size_t)r * nr + i;
+      const double psiT0 = vDT0[i] * JP;               // DT0_theta (Matrix_Operators.py:170-172)
+      if (Lb) {
+        // Step_Python right-hand sides (Main.py:271, 276, 280)
+        double grv = 0.0;
+        if (r <= K - 2) {
+          const double v = Ra * sT[rl + 1][i] - Ras * sS[rl + 1][i];
+          grv = ((double)(-(r + 1)) * vGR[i]) * v;     // kGR_RT row r <- cosine row r+1
+        }
+        Lb[o] = A2 + dtpr * grv;
+        Lb[N + o] = vRSQ[i] * sT[rl][i] - a.dt * psiT0;
+        Lb[2 * N + o] = vRSQ[i] * sS[rl][i] - a.dt * psiT0;
       }
-      if (r == 0) {
-        S9[1 * nK + row] = 0.0;
-        S9[2 * nK + row] = 0.0;
-        S9[5 * nK + row] = 0.0;
-        S9[6 * nK + row] = 0.0;
+      if (Th) {
+        Th[o] = JP;
+        Th[N + o] = psiT0;
+        Th[2 * N + o] = A2;
+        Th[3 * N + o] = Om;
+      }
+      sCd[0][rl][i] = Om;   // same thread read these entries above
+      sCd[1][rl][i] = JP;
+    }
+    lds_barrier();
+
+    if (a.slabs) {
+      const size_t nK = (size_t)nr * K;
+      R* S7 = reinterpret_cast<R*>(a.slabs) + (size_t)b * 7 * nK;
+#pragma unroll
+      for (int it = 0; it < kOutIters; ++it) {
+        const int idx = tx + 320 * it;
+        const int i = idx / kTile, rl = idx - i * kTile;
+        const int r = k0 + rl;
+        if (idx >= nr * kTile || r >= K) continue;
+        const size_t row = (size_t)i * K;
+        S7[row + r] = (R)sCd[1][rl][i];
+        S7[3 * nK + row + r] = (R)sCr[1][rl][i];
+        S7[4 * nK + row + r] = (R)sCr[2][rl][i];
+        S7[5 * nK + row + r] = (R)sT[rl][i];
+        S7[6 * nK + row + r] = (R)sS[rl][i];
+        if (r + 1 < K) {   // sine rows move to the sinusoid index r+1 (Matrix_Operators.py:734-737)
+          S7[nK + row + r + 1] = (R)sCr[0][rl][i];
+          S7[2 * nK + row + r + 1] = (R)sCd[0][rl][i];
+        }
+        if (r == 0) {
+          S7[nK + row] = (R)0;
+          S7[2 * nK + row] = (R)0;
+        }
+      }
+    }
+    if (a.slab9) {
+      const size_t nK = (size_t)nr * K;
+      double* S9 = a.slab9 + (size_t)b * 9 * nK;
+#pragma unroll
+      for (int it = 0; it < kOutIters; ++it) {
+        const int idx = tx + 320 * it;
+        const int i = idx / kTile, rl = idx - i * kTile;
+        const int r = k0 + rl;
+        if (idx >= nr * kTile || r >= K) continue;
+        const size_t row = (size_t)i * K;
+        // order: JT, kDpsi, komega, DT, DC, omega, Dpsi, kT, kC (Matrix_Operators.py:740)
+        S9[row + r] = sCd[1][rl][i];
+        S9[3 * nK + row + r] = sCr[1][rl][i];
+        S9[4 * nK + row + r] = sCr[2][rl][i];
+        S9[7 * nK + row + r] = (double)(-r) * sT[rl][i];
+        S9[8 * nK + row + r] = (double)(-r) * sS[rl][i];
+        if (r + 1 < K) {
+          const double ks = r + 1.0;
+          S9[1 * nK + row + r + 1] = ks * sCr[0][rl][i];
+          S9[2 * nK + row + r + 1] = ks * sCd[0][rl][i];
+          S9[5 * nK + row + r + 1] = sCd[0][rl][i];
+          S9[6 * nK + row + r + 1] = sCr[0][rl][i];
+        }
+        if (r == 0) {
+          S9[1 * nK + row] = 0.0;
+          S9[2 * nK + row] = 0.0;
+          S9[5 * nK + row] = 0.0;
+          S9[6 * nK + row] = 0.0;
+        }
       }
     }
   }
 }
 
+// Instances per spectral workgroup.  A workgroup overlaps the loads of its
+// next instance with the work on the current one, so runs of up to four
+// instances pay off once there are enough (tile, instance) pairs to keep
+// every CU busy; measured on the 512-mode, 39-point class at 64 and 256
+// instances: four per workgroup beat one, two and eight.
+static int spectral_instances(int tiles, int B) {
+  const long nb = ((long)tiles * B) / 512;
+  return nb < 1 ? 1 : (nb > kMaxSpectralInstances ? kMaxSpectralInstances : (int)nb);
+}
+
 template <typename R>
-static int launch_spectral_r(const Plan& p, const SpectralArgs& a, int B, hipStream_t s) {
-  const dim3 grid((p.K + kTile - 1) / kTile, B);
+static int launch_spectral_r(const Plan& p, SpectralArgs a, int B, hipStream_t s) {
+  const int tiles = (p.K + kTile - 1) / kTile;
+  a.B = B;
+  a.nb = spectral_instances(tiles, B);
+  const dim3 grid(tiles, (B + a.nb - 1) / a.nb);
   switch (p.NRP) {
     case 16: hipLaunchKernelGGL((k_spectral<R, 16>), grid, dim3(320), 0, s, a); break;
     case 32: hipLaunchKernelGGL((k_spectral<R, 32>), grid, dim3(320), 0, s, a); break;
