@@ -408,6 +408,15 @@ __device__ cplx<R>* fft_run_w(cplx<R>* a, cplx<R>* b, const FftShape& sh,
 // and the buffer is not restrict-qualified, so the compiler keeps every read
 // of a pass ahead of its writes.  ip_shape_ok checks the chunk condition.
 // ---------------------------------------------------------------------------
+// Padded LDS index of element x of an in-place transform buffer: one empty
+// slot after every 2^PS elements (PS = 0: no padding).  With 2^PS elements
+// per 128 bytes, the strided writes of the early Stockham passes (stride
+// RADIX * Ns elements) fall in different banks.
+template <int PS>
+__host__ __device__ constexpr int ip_pad(int x) {
+  return PS ? x + (x >> PS) : x;
+}
+
 template <int RADIX>
 struct IpChunk {
   static constexpr int U = RADIX >= 5 ? 2 : (RADIX == 4 ? 3 : 4);
@@ -427,7 +436,7 @@ inline bool ip_shape_ok(const FftShape& sh) {
   return true;
 }
 
-template <typename R, int RADIX>
+template <typename R, int RADIX, int PS>
 __device__ __forceinline__ void fft_pass_ip(cplx<R>* buf, int L, int Ns,
                                             const cplx<R>* __restrict__ tw, bool inv, int lane) {
   constexpr int U = IpChunk<RADIX>::U;
@@ -444,7 +453,7 @@ __device__ __forceinline__ void fft_pass_ip(cplx<R>* buf, int L, int Ns,
     const int k = j - jq * Ns;
     base[u] = jq * Ns * RADIX + k;
 #pragma unroll
-    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[j + s * nb];
+    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[ip_pad<PS>(j + s * nb)];
 #pragma unroll
     for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, s * k * st, inv);
   }
@@ -461,27 +470,98 @@ __device__ __forceinline__ void fft_pass_ip(cplx<R>* buf, int L, int Ns,
   for (int u = 0; u < U; ++u)
     if (lane + 64 * u < nb)
 #pragma unroll
-      for (int s = 0; s < RADIX; ++s) buf[base[u] + s * Ns] = v[u][s];
+      for (int s = 0; s < RADIX; ++s) buf[ip_pad<PS>(base[u] + s * Ns)] = v[u][s];
 }
 
 // All passes of one transform in place by one wavefront (shape checked by
-// ip_shape_ok on the host).
-template <typename R>
+// ip_shape_ok on the host); the buffer is indexed through ip_pad<PS>.
+template <typename R, int PS = 0>
 __device__ void fft_run_ip(cplx<R>* buf, const FftShape& sh, const cplx<R>* __restrict__ tw,
                            bool inv, int lane) {
   int Ns = 1;
   for (int p = 0; p < sh.npass; ++p) {
     const int r = sh.radix[p];
     if (r == 4)
-      fft_pass_ip<R, 4>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 4, PS>(buf, sh.L, Ns, tw, inv, lane);
     else if (r == 2)
-      fft_pass_ip<R, 2>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 2, PS>(buf, sh.L, Ns, tw, inv, lane);
     else if (r == 3)
-      fft_pass_ip<R, 3>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 3, PS>(buf, sh.L, Ns, tw, inv, lane);
     else
-      fft_pass_ip<R, 5>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 5, PS>(buf, sh.L, Ns, tw, inv, lane);
     wave_lds_sync();
     Ns *= r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The in-place transform with the length a compile-time constant: the pass
+// schedule is make_shape's (radix 4 first, then 2, 3, 5), and every stride,
+// chunk count and quotient of a pass folds into the instruction stream.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ip_radix(int L, int p) {
+  const int pref[4] = {4, 2, 3, 5};
+  int n = L, q = 0;
+  for (int i = 0; i < 4; ++i)
+    while (n % pref[i] == 0) {
+      if (q == p) return pref[i];
+      n /= pref[i];
+      ++q;
+    }
+  return 0;   // no pass p (or a factor above 5)
+}
+
+__host__ __device__ constexpr int ip_npass(int L) {
+  int q = 0;
+  while (ip_radix(L, q) != 0) ++q;
+  return q;
+}
+
+template <typename R, int L, int RADIX, int NS, int PS>
+__device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restrict__ tw,
+                                            bool inv, int lane) {
+  constexpr int NB = L / RADIX;                 // butterflies
+  constexpr int ST = L / (NS * RADIX);          // twiddle stride
+  constexpr int U = (NB + 63) / 64;             // chunks of 64 butterflies
+  static_assert(U <= IpChunk<RADIX>::U, "in-place chunk exceeds the register budget");
+  cplx<R> v[U][RADIX], w[U][RADIX];
+  int base[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int jr = lane + 64 * u;
+    const int j = (NB % 64 == 0 || jr < NB) ? jr : NB - 1;
+    const int jq = j / NS;
+    const int k = j - jq * NS;
+    base[u] = jq * NS * RADIX + k;
+#pragma unroll
+    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[ip_pad<PS>(j + s * NB)];
+#pragma unroll
+    for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, s * k * ST, inv);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+#pragma unroll
+    for (int s = 1; s < RADIX; ++s) v[u][s] = cmul(v[u][s], w[u][s]);
+    if (RADIX == 2) dft2(v[u]);
+    if (RADIX == 3) dft3(v[u], inv);
+    if (RADIX == 4) dft4(v[u], inv);
+    if (RADIX == 5) dft5(v[u], inv);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+    if (NB % 64 == 0 || lane + 64 * u < NB)
+#pragma unroll
+      for (int s = 0; s < RADIX; ++s) buf[ip_pad<PS>(base[u] + s * NS)] = v[u][s];
+}
+
+template <typename R, int L, int PS, int P = 0, int NS = 1>
+__device__ __forceinline__ void fft_run_ct(cplx<R>* buf, const cplx<R>* __restrict__ tw, bool inv,
+                                           int lane) {
+  if constexpr (P < ip_npass(L)) {
+    constexpr int RADIX = ip_radix(L, P);
+    fft_pass_ct<R, L, RADIX, NS, PS>(buf, tw, inv, lane);
+    wave_lds_sync();
+    fft_run_ct<R, L, PS, P + 1, NS * RADIX>(buf, tw, inv, lane);
   }
 }
 

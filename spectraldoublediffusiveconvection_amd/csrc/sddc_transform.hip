@@ -609,7 +609,8 @@ __global__ __launch_bounds__(320) void k_nonlinear_w(FftShape sh, const cplx<R>*
 // 64*NK modes).  Same work split and arithmetic as k_nonlinear_w, with three
 // changes that cut LDS from 12 L to 8 L complex values for F(X, X) (three
 // workgroups per CU at L = 768 in float):
-//  * each transform runs in place in one buffer (fft_run_ip);
+//  * each transform runs in place in one buffer (fft_run_ct: the pass
+//    schedule fixed at compile time, the buffer padded against bank conflicts);
 //  * the slab rows are not staged in LDS: lane l holds modes l + 64j in
 //    registers, the mirrored coefficient at index L - k needed by the
 //    Makhoul packing comes from lane (64 - l) mod 64 by a cross-lane read;
@@ -620,7 +621,7 @@ __global__ __launch_bounds__(320) void k_nonlinear_w(FftShape sh, const cplx<R>*
 // Loads field pair p of a slab set (one wavefront), scales both series by a
 // power of two and writes their packed, rotated spectrum W_k into A.  sinv
 // receives the two unscaling factors.
-template <typename R, int NC>
+template <typename R, int NC, int PS>
 __device__ __forceinline__ void ip_pack_pair(const SlabRows<R>& sg, int p, cplx<R>* A,
                                              const cplx<R>* __restrict__ rot_g, R* sinv,
                                              int lane) {
@@ -697,7 +698,7 @@ __device__ __forceinline__ void ip_pack_pair(const SlabRows<R>& sg, int p, cplx<
     const R au = us ? mu[c] : dcu, av = vs ? mv[c] : dcv;
     const R aum = k ? (us ? dcu : mu[c]) : (R)0;
     const R avm = k ? (vs ? dcv : mv[c]) : (R)0;
-    A[k] = cmul(C{au + avm, av - aum}, rr[c]);
+    A[ip_pad<PS>(k)] = cmul(C{au + avm, av - aum}, rr[c]);
   }
 }
 
@@ -724,7 +725,7 @@ struct IpTwiddles {
 // The second phase for wavefront p in {0, 1}: products o (registers) scaled
 // to unit magnitude, written to O, forward FFT in place, projection onto the
 // K modes and stores of output rows 2p, 2p+1.
-template <typename R, int NC>
+template <typename R, int NC, int PS>
 __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O, int p,
                                                    const cplx<R>* tw, const FftShape& sh,
                                                    const cplx<R>* __restrict__ rot_g,
@@ -749,9 +750,9 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
   e1 = __ockl_wfred_max_i32(e1);
   const R s0 = pow2_scale_e<R>(e0), s1 = pow2_scale_e<R>(e1);
 #pragma unroll
-  for (int c = 0; c < NC; ++c) O[lane + 64 * c] = {o[c].x * s0, o[c].y * s1};
+  for (int c = 0; c < NC; ++c) O[ip_pad<PS>(lane + 64 * c)] = {o[c].x * s0, o[c].y * s1};
   wave_lds_sync();
-  fft_run_ip<R>(O, sh, tw, false, lane);
+  fft_run_ct<R, L, PS>(O, tw, false, lane);
   const R i0 = (R)1 / s0, i1 = (R)1 / s1, invL = (R)1 / (R)L;
   R* f0 = F + ((size_t)(b * 4 + 2 * p) * nr + i) * K;
   R* f1 = F + ((size_t)(b * 4 + 2 * p + 1) * nr + i) * K;
@@ -762,8 +763,8 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
     const int ka = (p == 0) ? L - k : k;
     const int kam = ka < L ? ka : ka - L;
     const int kb = (L - ka) < L ? L - ka : L - ka - L;
-    za[j] = O[kam];
-    zb[j] = O[kb];
+    za[j] = O[ip_pad<PS>(kam)];
+    zb[j] = O[ip_pad<PS>(kb)];
   }
 #pragma unroll
   for (int j = 0; j < NK; ++j) {
@@ -790,14 +791,21 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
   }
 }
 
+// Bank padding of the F(X, X) kernel's transform buffers: one slot per
+// 128 bytes of complex values.
+template <typename R>
+constexpr int kIpPad = sizeof(R) == 4 ? 4 : 3;
+
 // F(X, X): five wavefronts, wavefront p synthesises field pair p.
-// LDS: twiddles (L), five pair buffers (5 L), two product buffers (2 L).
+// LDS: twiddles (L), five pair buffers and two product buffers of LP
+// (= L padded) values each.
 template <typename R, int NC>
 __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
                                                       const cplx<R>* __restrict__ rot_g, int nr,
                                                       const R* __restrict__ slabs,
                                                       R* __restrict__ F) {
   constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
+  constexpr int PS = kIpPad<R>, LP = ip_pad<PS>(L);
   static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ R sinv[10];   // unscaling factors of the nine fields
@@ -809,29 +817,29 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
   IpTwiddles<R, NC, 320> twr;
   twr.load(tw_g);
-  C* A = buf + (size_t)p * L;
-  ip_pack_pair<R, NC>(sg, p, A, rot_g, sinv + 2 * p, lane);
+  C* A = buf + (size_t)p * LP;
+  ip_pack_pair<R, NC, PS>(sg, p, A, rot_g, sinv + 2 * p, lane);
   twr.store(tw);
   __syncthreads();   // twiddles in LDS
-  fft_run_ip<R>(A, sh, tw, true, lane);
+  fft_run_ct<R, L, PS>(A, tw, true, lane);
   __syncthreads();
   if (p >= 2) return;
 
   // products of the grid fields (Matrix_Operators.py:791-793)
   const C* G0 = buf;
-  const C* G1 = buf + L;
-  const C* G2 = buf + 2 * L;
-  const C* G3 = buf + 3 * L;
-  const C* G4 = buf + 4 * L;
+  const C* G1 = buf + LP;
+  const C* G2 = buf + 2 * LP;
+  const C* G3 = buf + 3 * LP;
+  const C* G4 = buf + 4 * LP;
   R is[9];
 #pragma unroll
   for (int f = 0; f < 9; ++f) is[f] = sinv[f];
   C o[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const int q = lane + 64 * c;
+    const int q = lane + 64 * c, qp = ip_pad<PS>(q);
     const R gs = (R)grid_sign(q, L);
-    const C w0 = G0[q], w1 = G1[q], w2 = G2[q];
+    const C w0 = G0[qp], w1 = G1[qp], w2 = G2[qp];
     const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
     const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
     if (p == 0) {
@@ -840,13 +848,14 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
       o[c] = {gs * (JT * Om), gs * (kDpsi * Om + Dpsi * kOm)};
     } else {
       // Matrix_Operators.py:792-793
-      const C w3 = G3[q], w4 = G4[q];
+      const C w3 = G3[qp], w4 = G4[qp];
       const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
               DC = w4.x * is[8];
       o[c] = {JT * DT - Dpsi * kT, JT * DC - Dpsi * kC};
     }
   }
-  ip_forward_project<R, NC>(o, buf + (size_t)(5 + p) * L, p, tw, sh, rot_g, F, b, i, nr, lane);
+  ip_forward_project<R, NC, PS>(o, buf + (size_t)(5 + p) * LP, p, tw, sh, rot_g, F, b, i, nr,
+                                lane);
 }
 
 // F(X, dv) + F(dv, X) (NLIN_DFX, Matrix_Operators.py:807-898): ten
@@ -873,10 +882,10 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
   IpTwiddles<R, NC, 640> twr;
   twr.load(tw_g);
   C* A = buf + (size_t)q * L;
-  ip_pack_pair<R, NC>(sg, p, A, rot_g, sinv + (q < 5 ? 0 : 10) + 2 * p, lane);
+  ip_pack_pair<R, NC, 0>(sg, p, A, rot_g, sinv + (q < 5 ? 0 : 10) + 2 * p, lane);
   twr.store(tw);
   __syncthreads();   // twiddles in LDS
-  fft_run_ip<R>(A, sh, tw, true, lane);
+  fft_run_ct<R, L, 0>(A, tw, true, lane);
   __syncthreads();
   if (q >= 2) return;
 
@@ -924,7 +933,8 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
       o[c] = {tx, ty};
     }
   }
-  ip_forward_project<R, NC>(o, buf + (size_t)(10 + q) * L, q, tw, sh, rot_g, F, b, i, nr, lane);
+  ip_forward_project<R, NC, 0>(o, buf + (size_t)(10 + q) * L, q, tw, sh, rot_g, F, b, i, nr,
+                               lane);
 }
 
 // Columns per lane NC of the in-place kernel for this plan, 0 when its
@@ -953,17 +963,26 @@ int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& curren
   return SDDC_OK;
 }
 
+// Dynamic LDS of the F(X, X) in-place kernel: twiddles (L) and seven
+// padded transform buffers.
+template <typename R>
+static size_t nl_ip_lds(int L) {
+  return ((size_t)L + 7 * (size_t)ip_pad<kIpPad<R>>(L)) * sizeof(cplx<R>);
+}
+
 int nonlinear_lds_bytes(const Plan& p, int precision, int /*dfx*/) {
   const size_t c = precision == SDDC_MIXED ? sizeof(float2) : sizeof(double2);
-  // in place: twiddles + 5 pair + 2 product buffers; else 2 x 5 working
-  // arrays + twiddles + rotations
-  return (int)((nl_inplace_nc(p) ? 8 : 12) * (size_t)p.M * c);
+  // in place: twiddles + 5 pair + 2 product buffers (padded); else 2 x 5
+  // working arrays + twiddles + rotations
+  if (nl_inplace_nc(p))
+    return (int)(precision == SDDC_MIXED ? nl_ip_lds<float>(p.M) : nl_ip_lds<double>(p.M));
+  return (int)(12 * (size_t)p.M * c);
 }
 
 template <typename R, int NC>
 static int launch_nl_ip(const Plan& p, int B, const void* slabs, void* F, hipStream_t s,
                         const cplx<R>* tw, const cplx<R>* rot) {
-  const size_t lds = 8 * (size_t)(64 * NC) * sizeof(cplx<R>);
+  const size_t lds = nl_ip_lds<R>(64 * NC);
   static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_ip<R, NC>, lds, cur)) return rc;
   hipLaunchKernelGGL((k_nonlinear_ip<R, NC>), dim3(p.nr, B), dim3(320), lds, s, p.tabM.shape, tw,
