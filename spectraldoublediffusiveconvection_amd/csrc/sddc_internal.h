@@ -106,6 +106,7 @@ struct Plan {
   int seg_m = 0, nseg = 0;     // rows per chain segment, segments per chain
   double* chain_L[3] = {nullptr, nullptr, nullptr};   // block inverses, K x NRP x NRS
   double* chain_DL = nullptr;                         // D2 Linv_j of the A4 solve, per mode
+  double* chain_D2 = nullptr;                         // D2 of the A4 solve (one block)
   double* chain_Q[3] = {nullptr, nullptr, nullptr};   // row corrections (nseg > 1)
   double* chain_E[3] = {nullptr, nullptr, nullptr};   // segment exit maps (nseg > 1)
   double* chain_Lm[3] = {nullptr, nullptr, nullptr};  // MFMA-packed block inverses

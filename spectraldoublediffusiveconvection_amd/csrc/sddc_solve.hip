@@ -45,6 +45,7 @@ namespace sddc {
 constexpr int kDepth = 2;    // chain rows of operator data in flight per lane
 constexpr int kDepthE = 2;   // segment exit maps in flight per lane (join)
 constexpr int kDepthM = 4;   // rows in flight in the MFMA kernel (few waves: deep prefetch)
+constexpr int kDepthMixed = 2;   // rows in flight in the mixed-width kernel
 
 struct ChainArgs {
   const double* G;          // right-hand sides, field f of instance b at b*gstride+goff[f]
@@ -52,6 +53,7 @@ struct ChainArgs {
   double* P;                // psi suffix sums [B][N] or null
   const double* Lq[3];      // packed block inverses per field, block index jj = K-1-row
   const double* DLq;        // packed D2 Linv_j of the A4 solve, per mode
+  const double* D2q;        // packed D2 of the A4 solve (one block, all modes)
   const double* Qq[3];      // packed corrections, index jj
   const double* Eq[3];      // packed segment exit maps, index chain*nseg + seg
   const double* Lm[3];      // MFMA-packed block inverses (16-instance kernel)
@@ -154,7 +156,7 @@ __device__ __forceinline__ Inst<NB> instances(const ChainArgs& a, int field, int
 // ---------------------------------------------------------------------------
 // k_chain_local: one segment with zero entry state.
 // ---------------------------------------------------------------------------
-template <int NRS, int NB>
+template <int NRS, int NB, int KD = kDepth>
 __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int start, int k0,
                                            int k1, const Inst<NB>& in,
                                            double (*xs)[NB][NRS + 64], double* exit_seg) {
@@ -169,8 +171,8 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
   const double dtf = a.dtf[field];
   const double* lb = a.Lq[field] + w * S::NQ * 128 + 2 * lane;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
-  double2 blk[kDepth][S::NQ];
-  double gv[kDepth][NB];
+  double2 blk[KD][S::NQ];
+  double gv[KD][NB];
   auto fetch = [&](int k, double2 (&bq)[S::NQ], double (&g)[NB]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
     load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
@@ -178,14 +180,14 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
     for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
   };
 #pragma unroll
-  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], gv[d]);
+  for (int d = 0; d < KD; ++d) fetch(k0 + d, blk[d], gv[d]);
   drain_loads();
   double b[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) b[t] = 0.0;
-  for (int kb = k0; kb < k1; kb += kDepth) {
+  for (int kb = k0; kb < k1; kb += KD) {
 #pragma unroll
-    for (int d = 0; d < kDepth; ++d) {
+    for (int d = 0; d < KD; ++d) {
       const int k = kb + d;
       const bool live = k < k1;
       const int j = start - 2 * (live ? k : k1 - 1);
@@ -195,18 +197,18 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
         const double rhs = gv[d][t] - cj * b[t];
-        xs[d & 1][t][wp] = rhs;
+        xs[k & 1][t][wp] = rhs;
       }
       lds_barrier();
       const double alpha = 2.0 * dtf * j;
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[d & 1][t][kq * S::CPL]));
+        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]));
         double* dst = (live && rv) ? in.x[t] + (size_t)j * nr + r : sink;
         *dst = f;
         b[t] = live ? b[t] + alpha * f : b[t];
       }
-      fetch(k + kDepth, blk[d], gv[d]);
+      fetch(k + KD, blk[d], gv[d]);
     }
   }
 #pragma unroll
@@ -216,7 +218,7 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
   }
 }
 
-template <int NRS, int NB>
+template <int NRS, int NB, int KD = kDepth>
 __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, int k1,
                                          const Inst<NB>& in, double (*xs)[NB][NRS + 64],
                                          double* exit_seg) {
@@ -234,8 +236,8 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
   const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lane;
   const double* mb = a.DLq + w * S::NQ * 128 + 2 * lane;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
-  double2 blk[kDepth][S::NQ], mblk[kDepth][S::NQ];
-  double gv[kDepth][NB];
+  double2 blk[KD][S::NQ], mblk[KD][S::NQ];
+  double gv[KD][NB];
   auto fetch = [&](int k, double2 (&bq)[S::NQ], double2 (&mq)[S::NQ], double (&g)[NB]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
     load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
@@ -244,14 +246,14 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
     for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
   };
 #pragma unroll
-  for (int d = 0; d < kDepth; ++d) fetch(k0 + d, blk[d], mblk[d], gv[d]);
+  for (int d = 0; d < KD; ++d) fetch(k0 + d, blk[d], mblk[d], gv[d]);
   drain_loads();
   double fe[NB], bfe[NB], dfe[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) fe[t] = bfe[t] = dfe[t] = 0.0;
-  for (int kb = k0; kb < k1; kb += kDepth) {
+  for (int kb = k0; kb < k1; kb += KD) {
 #pragma unroll
-    for (int d = 0; d < kDepth; ++d) {
+    for (int d = 0; d < KD; ++d) {
       const int k = kb + d;
       const bool live = k < k1;
       const int row = start - 2 * (live ? k : k1 - 1);
@@ -270,13 +272,13 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
         *pdst = fe[t];
         const double l1fe = dfe[t] + bj * (ir4 * fe[t]);
         const double bb = (dta * bjt) * (l1fe + ir4 * bfe[t]) - bjt * (ir2 * fe[t]);
-        xs[d & 1][t][wp] = gv[d][t] + bb;
+        xs[k & 1][t][wp] = gv[d][t] + bb;
       }
       lds_barrier();
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[d & 1][t][kq * S::CPL]));
-        const double df = quad_sum(pdot<S::NQ>(mblk[d], &xs[d & 1][t][kq * S::CPL]));
+        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]));
+        const double df = quad_sum(pdot<S::NQ>(mblk[d], &xs[k & 1][t][kq * S::CPL]));
         double* dst = (live && rv) ? in.x[t] + (size_t)row * nr + r : sink;
         *dst = f;
         const double nbfe = bfe[t] + bj * f + bjt * fe[t];
@@ -284,7 +286,96 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
         fe[t] = live ? fe[t] + f : fe[t];
         dfe[t] = live ? dfe[t] + df : dfe[t];
       }
-      fetch(k + kDepth, blk[d], mblk[d], gv[d]);
+      fetch(k + KD, blk[d], mblk[d], gv[d]);
+    }
+  }
+  const size_t off = exit_seg - a.EX;
+#pragma unroll
+  for (int t = 0; t < NB; ++t) {
+    double* df = rv ? in.ex[t] + off + r : sink;
+    double* db = rv ? in.ex[t] + off + S::NRP + r : sink;
+    *df = fe[t];
+    *db = bfe[t];
+  }
+}
+
+// The A4 chain with two exchanges per row: D2 f_e comes from f_e itself,
+// with D2 (the same block for every mode) held in registers, so a row
+// streams one operator block instead of two.  Same recurrence and results
+// as local_a4 up to summation order.
+template <int NRS, int NB, int KD = kDepth>
+__device__ __forceinline__ void local_a4x(const ChainArgs& a, int start, int k0, int k1,
+                                          const Inst<NB>& in, double (*xs)[NB][NRS + 64],
+                                          double* exit_seg) {
+  using S = CShape<NRS>;
+  const int K = a.K, nr = a.nr;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int r = 16 * w + (lane >> 2), kq = lane & 3;
+  const bool rv = r < nr;
+  const int rr = rv ? r : nr - 1;
+  const bool writer = kq == 0 && r < NRS;
+  const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
+  const double dta = a.dtf[0];
+  const double ir4 = a.vecs[V_IR4A4 * S::NRP + rr];
+  const double ir2 = a.vecs[V_IR2A4 * S::NRP + rr];
+  double2 d2[S::NQ];
+  load_blk<S::NQ>(d2, a.D2q + w * S::NQ * 128 + 2 * lane);
+  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lane;
+  double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
+  double2 blk[KD][S::NQ];
+  double gv[KD][NB];
+  auto fetch = [&](int k, double2 (&bq)[S::NQ], double (&g)[NB]) {
+    const int j = start - 2 * (k < k1 ? k : k1 - 1);
+    load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
+  };
+#pragma unroll
+  for (int d = 0; d < KD; ++d) fetch(k0 + d, blk[d], gv[d]);
+  drain_loads();
+  double fe[NB], bfe[NB];
+#pragma unroll
+  for (int t = 0; t < NB; ++t) fe[t] = bfe[t] = 0.0;
+  for (int kb = k0; kb < k1; kb += KD) {
+#pragma unroll
+    for (int d = 0; d < KD; ++d) {
+      const int k = kb + d;
+      const bool live = k < k1;
+      const int row = start - 2 * (live ? k : k1 - 1);
+      // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt, j = row + 1:
+      // b = dt bjt (L1 f_e + IR4 bf_e) - bjt IR2 f_e with L1 = D2 + bj IR4;
+      // f = Linv (g + b); bf_e += bj f + bjt f_e; f_e += f.  f_e is also the
+      // same-parity suffix sum P of the row.  Buffer 0 carries f_e, buffer 1
+      // the right-hand side; the barrier after buffer 1 is written orders
+      // every read of buffer 0 before the next row overwrites it.
+      const double jp = (double)(row + 1);
+      const double bj = -jp * (jp + 1.0);
+      const double bjt = -2.0 * jp;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        double* pdst = (live && rv) ? in.p[t] + (size_t)row * nr + r : sink;
+        *pdst = fe[t];
+        xs[0][t][wp] = fe[t];
+      }
+      lds_barrier();
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double d2fe = quad_sum(pdot<S::NQ>(d2, &xs[0][t][kq * S::CPL]));
+        const double l1fe = d2fe + bj * (ir4 * fe[t]);
+        const double bb = (dta * bjt) * (l1fe + ir4 * bfe[t]) - bjt * (ir2 * fe[t]);
+        xs[1][t][wp] = gv[d][t] + bb;
+      }
+      lds_barrier();
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[1][t][kq * S::CPL]));
+        double* dst = (live && rv) ? in.x[t] + (size_t)row * nr + r : sink;
+        *dst = f;
+        const double nbfe = bfe[t] + bj * f + bjt * fe[t];
+        bfe[t] = live ? nbfe : bfe[t];
+        fe[t] = live ? fe[t] + f : fe[t];
+      }
+      fetch(k + KD, blk[d], gv[d]);
     }
   }
   const size_t off = exit_seg - a.EX;
@@ -332,18 +423,23 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs 
     local_nab2<NRS, NB>(a, field, start, k0, k1, in, xs, exit_seg);
 }
 
-// k_chain_local with two widths in one launch (solve_instances = 3): the A4
-// chains, with two products per row, take 2 instances per workgroup and the
-// NAB2 chains 4.  The launch has ceil(B/2) groups; NAB2 workgroups past
-// ceil(B/4) return at once.  Register use is the larger of the two paths,
-// well below that of 4 instances on the A4 path.
-template <int NRS>
+// k_chain_local with two widths in one launch: the A4 chains take NBA
+// instances per workgroup (X2: the two-exchange form, one streamed block per
+// row), the NAB2 chains NBN.  The launch has ceil(B / min(NBA, NBN)) groups;
+// workgroups past the end of the batch for their width return at once.
+//   solve_instances 3: NBA 2, NBN 4 (one exchange per A4 row)
+//   solve_instances 5: NBA 2, NBN 4, X2
+// Measured at N_fm = 512, N_r = 40 (B = 64 and 256): 5 is the fastest in
+// the sub-batched graph (it streams a third of 3's A4 bytes); four A4 or
+// eight NAB2 instances per workgroup were slower than both.
+template <int NRS, int NBA, int NBN, bool X2>
 __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(ChainArgs a) {
   using S = CShape<NRS>;
-  __shared__ __align__(16) double xs[2][4][NRS + 64];
+  constexpr int NBX = NBA > NBN ? NBA : NBN;
+  __shared__ __align__(16) double xs[2][NBX][NRS + 64];
   const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y;
   if (!((a.fields_mask >> field) & 1)) return;
-  if (field != 0 && (int)blockIdx.z * 4 >= a.B) return;
+  if ((int)blockIdx.z * (field == 0 ? NBA : NBN) >= a.B) return;
   const int start = a.K - 1 - chain;
   const int n = start / 2 + 1;
   const int k0 = seg * a.m;
@@ -352,7 +448,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
   double* exit_seg = a.EX + (size_t)seg * 2 * S::NRP;
   const bool sym_zero = a.symmetric && (field == 0 ? chain == 1 : chain == 0);
   if (field == 0) {
-    const Inst<2> in = instances<2>(a, field, chain);
+    const Inst<NBA> in = instances<NBA>(a, field, chain);
     if (sym_zero) {   // Eq_SYM: this chain is zero (Matrix_Operators.py:1166)
       const int lane = threadIdx.x & 63;
       const int r = 16 * (threadIdx.x >> 6) + (lane >> 2);
@@ -360,18 +456,21 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
         for (int k = k0; k < k1; ++k) {
           const size_t o = (size_t)(start - 2 * k) * a.nr + r;
 #pragma unroll
-          for (int t = 0; t < 2; ++t) {
+          for (int t = 0; t < NBA; ++t) {
             in.x[t][o] = 0.0;
             in.p[t][o] = 0.0;
           }
         }
       return;
     }
-    local_a4<NRS, 2>(a, start, k0, k1, in, reinterpret_cast<double (*)[2][NRS + 64]>(xs),
-                     exit_seg);
+    auto* xa = reinterpret_cast<double (*)[NBA][NRS + 64]>(xs);
+    if constexpr (X2)
+      local_a4x<NRS, NBA>(a, start, k0, k1, in, xa, exit_seg);
+    else
+      local_a4<NRS, NBA, kDepthMixed>(a, start, k0, k1, in, xa, exit_seg);
     return;
   }
-  const Inst<4> in = instances<4>(a, field, chain);
+  const Inst<NBN> in = instances<NBN>(a, field, chain);
   if (sym_zero) {   // Eq_SYM: this chain is zero (Matrix_Operators.py:1054)
     const int lane = threadIdx.x & 63;
     const int r = 16 * (threadIdx.x >> 6) + (lane >> 2);
@@ -379,11 +478,12 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
       for (int k = k0; k < k1; ++k) {
         const size_t o = (size_t)(start - 2 * k) * a.nr + r;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) in.x[t][o] = 0.0;
+        for (int t = 0; t < NBN; ++t) in.x[t][o] = 0.0;
       }
     return;
   }
-  local_nab2<NRS, 4>(a, field, start, k0, k1, in, xs, exit_seg);
+  local_nab2<NRS, NBN, kDepthMixed>(a, field, start, k0, k1, in,
+                                    reinterpret_cast<double (*)[NBN][NRS + 64]>(xs), exit_seg);
 }
 
 // ---------------------------------------------------------------------------
@@ -820,14 +920,16 @@ static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
   return SDDC_OK;
 }
 
-template <int NRS>
+template <int NRS, int NBA, int NBN, bool X2>
 static int launch_chain_mixed_t(const ChainArgs& a, int B, hipStream_t s) {
   using S = CShape<NRS>;
-  const dim3 grid(6, a.nseg, (B + 1) / 2), block(64 * S::WR);
-  hipLaunchKernelGGL((k_chain_local_mixed<NRS>), grid, block, 0, s, a);
+  constexpr int NBMIN = NBA < NBN ? NBA : NBN;
+  const dim3 grid(6, a.nseg, (B + NBMIN - 1) / NBMIN), block(64 * S::WR);
+  hipLaunchKernelGGL((k_chain_local_mixed<NRS, NBA, NBN, X2>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local_mixed")) return rc;
-  if (a.nseg > 1) {
-    hipLaunchKernelGGL((k_chain_join<NRS, 2>), grid, block, 0, s, a);
+  if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup
+    const dim3 gj(6, a.nseg, (B + 1) / 2);
+    hipLaunchKernelGGL((k_chain_join<NRS, 2>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
   }
   return SDDC_OK;
@@ -838,7 +940,8 @@ static int dispatch_nb(const ChainArgs& a, int NB, int B, hipStream_t s) {
   switch (NB) {
     case 1: return launch_chain_t<NRS, 1>(a, B, s);
     case 2: return launch_chain_t<NRS, 2>(a, B, s);
-    case 3: return launch_chain_mixed_t<NRS>(a, B, s);
+    case 3: return launch_chain_mixed_t<NRS, 2, 4, false>(a, B, s);
+    case 5: return launch_chain_mixed_t<NRS, 2, 4, true>(a, B, s);
     case 4: return launch_chain_t<NRS, 4>(a, B, s);
     default: return fail(SDDC_ERR_INVALID, "unsupported instances per workgroup");
   }
@@ -894,6 +997,7 @@ int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
     a.xoff[f] = xoff[f];
   }
   a.DLq = p.chain_DL;
+  a.D2q = p.chain_D2;
   a.DLm = p.chain_DLm;
   a.vecs = p.vecs;
   a.EX = p.chain_EX;
@@ -1028,6 +1132,11 @@ int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment) 
   for (int jj = 0; jj < K; ++jj)
     pack_quad(dl.data() + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
   if (int rc = upload_vec(&p.chain_DL, buf)) return rc;
+  {   // D2 itself, for the two-exchange A4 chains
+    std::vector<double> d2b((size_t)BLK);
+    pack_quad(ops->D2_A4, nr, nr, nr, NRP, NRS, d2b.data());
+    if (int rc = upload_vec(&p.chain_D2, d2b)) return rc;
+  }
   if (p.tune_solve_nbi == 16) {   // operands of the MFMA chain kernel
     for (int f = 0; f < 3; ++f) {
       for (int jj = 0; jj < K; ++jj)
@@ -1125,6 +1234,8 @@ void free_chain(Plan& p) {
   }
   if (p.chain_DL) (void)hipFree(p.chain_DL);
   p.chain_DL = nullptr;
+  if (p.chain_D2) (void)hipFree(p.chain_D2);
+  p.chain_D2 = nullptr;
   for (int f = 0; f < 3; ++f) {
     if (p.chain_Lm[f]) (void)hipFree(p.chain_Lm[f]);
     p.chain_Lm[f] = nullptr;
