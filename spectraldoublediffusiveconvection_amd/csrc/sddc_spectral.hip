@@ -470,10 +470,13 @@ template <typename R, int NRP>
 __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
   constexpr int NT = NRP / 16;   // <= 4: one row tile per wavefront
   constexpr int NK = NRP / 4;
-  __shared__ R sQ[kTile][NRP];
-  __shared__ R sR[kTile][NRP];
-  __shared__ R sTT[kTile][NRP];
-  __shared__ R sSS[kTile][NRP];
+  // rows padded to an odd length: the transposing fill below and the MFMA
+  // operand reads take one column of 16 rows, which then fall in different
+  // LDS banks
+  __shared__ R sQ[kTile][NRP + 1];
+  __shared__ R sR[kTile][NRP + 1];
+  __shared__ R sTT[kTile][NRP + 1];
+  __shared__ R sSS[kTile][NRP + 1];
   __shared__ R sDQ[kTile][NRP + 1];
   const int K = a.K, nr = a.nr, b = blockIdx.y, k0 = blockIdx.x * kTile;
   const size_t N = (size_t)K * nr;
