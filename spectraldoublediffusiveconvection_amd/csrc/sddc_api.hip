@@ -9,6 +9,7 @@
 
 #include <vector>
 
+#include "sddc_fft.h"
 #include "sddc_internal.h"
 
 
@@ -73,14 +74,35 @@ int build_tables(TableSet& t, int L) {
     tw32[k] = make_float2((float)tw[k].x, (float)tw[k].y);
     rot32[k] = make_float2((float)rot[k].x, (float)rot[k].y);
   }
+  // per-pass layout of the same values for the in-place transforms (entry
+  // L-1 is unused); its schedule is make_shape's for lengths of the factors
+  // 2, 3 and 5, the only ones the in-place kernels take
+  std::vector<double2> twp(L, make_double2(1.0, 0.0));
+  std::vector<float2> twp32(L, make_float2(1.0f, 0.0f));
+  if (ip_npass(L) > 0 && ip_twiddle_offset(L, ip_npass(L)) == L - 1) {
+    int ns = 1;
+    for (int q = 0; q < ip_npass(L); ++q) {
+      const int r = ip_radix(L, q), st = L / (ns * r), off = ip_twiddle_offset(L, q);
+      for (int s = 1; s < r; ++s)
+        for (int k = 0; k < ns; ++k) {
+          twp[off + (s - 1) * ns + k] = tw[s * k * st];
+          twp32[off + (s - 1) * ns + k] = tw32[s * k * st];
+        }
+      ns *= r;
+    }
+  }
   SDDC_HIP(hipMalloc(&t.tw64, L * sizeof(double2)));
   SDDC_HIP(hipMalloc(&t.rot64, L * sizeof(double2)));
   SDDC_HIP(hipMalloc(&t.tw32, L * sizeof(float2)));
   SDDC_HIP(hipMalloc(&t.rot32, L * sizeof(float2)));
+  SDDC_HIP(hipMalloc(&t.twp64, L * sizeof(double2)));
+  SDDC_HIP(hipMalloc(&t.twp32, L * sizeof(float2)));
   SDDC_HIP(hipMemcpy(t.tw64, tw.data(), L * sizeof(double2), hipMemcpyHostToDevice));
   SDDC_HIP(hipMemcpy(t.rot64, rot.data(), L * sizeof(double2), hipMemcpyHostToDevice));
   SDDC_HIP(hipMemcpy(t.tw32, tw32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
   SDDC_HIP(hipMemcpy(t.rot32, rot32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
+  SDDC_HIP(hipMemcpy(t.twp64, twp.data(), L * sizeof(double2), hipMemcpyHostToDevice));
+  SDDC_HIP(hipMemcpy(t.twp32, twp32.data(), L * sizeof(float2), hipMemcpyHostToDevice));
   return SDDC_OK;
 }
 
@@ -89,6 +111,8 @@ void free_tables(TableSet& t) {
   if (t.rot64) (void)hipFree(t.rot64);
   if (t.tw32) (void)hipFree(t.tw32);
   if (t.rot32) (void)hipFree(t.rot32);
+  if (t.twp64) (void)hipFree(t.twp64);
+  if (t.twp32) (void)hipFree(t.twp32);
   t = TableSet{};
 }
 

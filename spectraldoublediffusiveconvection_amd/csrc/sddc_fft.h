@@ -517,11 +517,24 @@ __host__ __device__ constexpr int ip_npass(int L) {
   return q;
 }
 
-template <typename R, int L, int RADIX, int NS, int PS>
+// Per-pass twiddle layout: pass P (radix R, NS points already combined)
+// holds W_L^(s k L/(NS R)) for s = 1..R-1, k = 0..NS-1 at
+// ip_twiddle_offset(L, P) + (s-1) NS + k; the passes together use L - 1
+// entries.
+__host__ __device__ constexpr int ip_twiddle_offset(int L, int P) {
+  int off = 0, ns = 1;
+  for (int q = 0; q < P; ++q) {
+    const int r = ip_radix(L, q);
+    off += (r - 1) * ns;
+    ns *= r;
+  }
+  return off;
+}
+
+template <typename R, int L, int RADIX, int NS, int PS, int TWO>
 __device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restrict__ tw,
                                             bool inv, int lane) {
   constexpr int NB = L / RADIX;                 // butterflies
-  constexpr int ST = L / (NS * RADIX);          // twiddle stride
   constexpr int U = (NB + 63) / 64;             // chunks of 64 butterflies
   static_assert(U <= IpChunk<RADIX>::U, "in-place chunk exceeds the register budget");
   cplx<R> v[U][RADIX], w[U][RADIX];
@@ -536,7 +549,7 @@ __device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restr
 #pragma unroll
     for (int s = 0; s < RADIX; ++s) v[u][s] = buf[ip_pad<PS>(j + s * NB)];
 #pragma unroll
-    for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, s * k * ST, inv);
+    for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, TWO + (s - 1) * NS + k, inv);
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
@@ -554,12 +567,13 @@ __device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restr
       for (int s = 0; s < RADIX; ++s) buf[ip_pad<PS>(base[u] + s * NS)] = v[u][s];
 }
 
+// All passes; tw is the per-pass twiddle table (ip_twiddle_offset).
 template <typename R, int L, int PS, int P = 0, int NS = 1>
 __device__ __forceinline__ void fft_run_ct(cplx<R>* buf, const cplx<R>* __restrict__ tw, bool inv,
                                            int lane) {
   if constexpr (P < ip_npass(L)) {
     constexpr int RADIX = ip_radix(L, P);
-    fft_pass_ct<R, L, RADIX, NS, PS>(buf, tw, inv, lane);
+    fft_pass_ct<R, L, RADIX, NS, PS, ip_twiddle_offset(L, P)>(buf, tw, inv, lane);
     wave_lds_sync();
     fft_run_ct<R, L, PS, P + 1, NS * RADIX>(buf, tw, inv, lane);
   }

@@ -85,6 +85,10 @@ struct TableSet {
   double2* rot64 = nullptr;
   float2* tw32 = nullptr;
   float2* rot32 = nullptr;
+  // the same twiddles laid out per pass of the shape (see ip_twiddle_offset):
+  // a pass reads its entries at consecutive addresses
+  double2* twp64 = nullptr;
+  float2* twp32 = nullptr;
 };
 
 // Per-size-class operator set and workspace.

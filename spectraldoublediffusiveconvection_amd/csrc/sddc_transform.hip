@@ -2,6 +2,7 @@
 // advection kernel (NLIN_FX / NLIN_DFX, Matrix_Operators.py:743-898) and the
 // radial rows of the kinetic energy (Main.py:71-134).
 #include <atomic>
+#include <type_traits>
 
 #include "sddc_fft.h"
 
@@ -797,10 +798,11 @@ template <typename R>
 constexpr int kIpPad = sizeof(R) == 4 ? 4 : 3;
 
 // F(X, X): five wavefronts, wavefront p synthesises field pair p.
-// LDS: twiddles (L), five pair buffers and two product buffers of LP
-// (= L padded) values each.
+// LDS: twiddles (L) and five pair buffers of LP (= L padded) values each;
+// once every product is in registers, wavefronts 0 and 1 reuse pair
+// buffers 3 and 4 for their forward transforms.
 template <typename R, int NC>
-__global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
+__global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
                                                       const cplx<R>* __restrict__ rot_g, int nr,
                                                       const R* __restrict__ slabs,
                                                       R* __restrict__ F) {
@@ -823,7 +825,6 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   __syncthreads();   // twiddles in LDS
   fft_run_ct<R, L, PS>(A, tw, true, lane);
   __syncthreads();
-  if (p >= 2) return;
 
   // products of the grid fields (Matrix_Operators.py:791-793)
   const C* G0 = buf;
@@ -837,6 +838,7 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   C o[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
+    if (p >= 2) break;
     const int q = lane + 64 * c, qp = ip_pad<PS>(q);
     const R gs = (R)grid_sign(q, L);
     const C w0 = G0[qp], w1 = G1[qp], w2 = G2[qp];
@@ -854,21 +856,25 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
       o[c] = {JT * DT - Dpsi * kT, JT * DC - Dpsi * kC};
     }
   }
-  ip_forward_project<R, NC, PS>(o, buf + (size_t)(5 + p) * LP, p, tw, sh, rot_g, F, b, i, nr,
+  __syncthreads();   // every read of the pair buffers is done
+  if (p >= 2) return;
+  ip_forward_project<R, NC, PS>(o, buf + (size_t)(3 + p) * LP, p, tw, sh, rot_g, F, b, i, nr,
                                 lane);
 }
 
 // F(X, dv) + F(dv, X) (NLIN_DFX, Matrix_Operators.py:807-898): ten
 // wavefronts, wavefront q < 5 synthesises pair q of X, q >= 5 pair q-5 of dv.
-// LDS: twiddles (L), ten pair buffers (10 L), two product buffers (2 L).
+// LDS: twiddles (L), ten padded pair buffers; wavefronts 0 and 1 reuse pair
+// buffers 3 and 4 for their forward transforms once the products are formed.
 template <typename R, int NC>
-__global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
+__global__ __launch_bounds__(640) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) void k_nonlinear_dfx_ip(FftShape sh,
                                                           const cplx<R>* __restrict__ tw_g,
                                                           const cplx<R>* __restrict__ rot_g,
                                                           int nr, const R* __restrict__ slabs,
                                                           const R* __restrict__ slabs2,
                                                           R* __restrict__ F) {
   constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
+  constexpr int PS = kIpPad<R>, LP = ip_pad<PS>(L);
   static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ R sinv[20];   // unscaling factors: fields of X, then of dv
@@ -881,13 +887,12 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
   const SlabRows<R> sg = slab_rows<R>(q < 5 ? slabs : slabs2, b, i, nr, K);
   IpTwiddles<R, NC, 640> twr;
   twr.load(tw_g);
-  C* A = buf + (size_t)q * L;
-  ip_pack_pair<R, NC, 0>(sg, p, A, rot_g, sinv + (q < 5 ? 0 : 10) + 2 * p, lane);
+  C* A = buf + (size_t)q * LP;
+  ip_pack_pair<R, NC, PS>(sg, p, A, rot_g, sinv + (q < 5 ? 0 : 10) + 2 * p, lane);
   twr.store(tw);
   __syncthreads();   // twiddles in LDS
-  fft_run_ct<R, L, 0>(A, tw, true, lane);
+  fft_run_ct<R, L, PS>(A, tw, true, lane);
   __syncthreads();
-  if (q >= 2) return;
 
   // Frechet derivative (Matrix_Operators.py:884-887): every term is (field of
   // X) * (field of dv), accumulated in the order of the pairs of dv
@@ -900,10 +905,11 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
   C o[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const int qq = lane + 64 * c;
+    if (q >= 2) break;
+    const int qq = lane + 64 * c, qp = ip_pad<PS>(qq);
     const R gs = (R)grid_sign(qq, L);
-    const C w0 = buf[qq], w1 = buf[L + qq], w2 = buf[2 * L + qq];
-    const C v0 = buf[5 * L + qq], v1 = buf[6 * L + qq], v2 = buf[7 * L + qq];
+    const C w0 = buf[qp], w1 = buf[LP + qp], w2 = buf[2 * LP + qp];
+    const C v0 = buf[5 * LP + qp], v1 = buf[6 * LP + qp], v2 = buf[7 * LP + qp];
     const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
     const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
     const R dJT = v0.x * id[0], dOm = gs * v0.y * id[1];
@@ -917,8 +923,8 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
       zy += dkDpsi * Om;
       o[c] = {gs * zx, gs * zy};
     } else {
-      const C w3 = buf[3 * L + qq], w4 = buf[4 * L + qq];
-      const C v3 = buf[8 * L + qq], v4 = buf[9 * L + qq];
+      const C w3 = buf[3 * LP + qp], w4 = buf[4 * LP + qp];
+      const C v3 = buf[8 * LP + qp], v4 = buf[9 * LP + qp];
       const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
               DC = w4.x * is[8];
       const R dkT = gs * v2.y * id[5], dDT = v3.x * id[6], dkC = gs * v3.y * id[7],
@@ -933,8 +939,10 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
       o[c] = {tx, ty};
     }
   }
-  ip_forward_project<R, NC, 0>(o, buf + (size_t)(10 + q) * L, q, tw, sh, rot_g, F, b, i, nr,
-                               lane);
+  __syncthreads();   // every read of the pair buffers is done
+  if (q >= 2) return;
+  ip_forward_project<R, NC, PS>(o, buf + (size_t)(3 + q) * LP, q, tw, sh, rot_g, F, b, i, nr,
+                                lane);
 }
 
 // Columns per lane NC of the in-place kernel for this plan, 0 when its
@@ -963,11 +971,11 @@ int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& curren
   return SDDC_OK;
 }
 
-// Dynamic LDS of the F(X, X) in-place kernel: twiddles (L) and seven
-// padded transform buffers.
+// Dynamic LDS of the in-place kernels: twiddles (L) and the padded pair
+// buffers (five for F(X, X), ten for the Frechet derivative).
 template <typename R>
-static size_t nl_ip_lds(int L) {
-  return ((size_t)L + 7 * (size_t)ip_pad<kIpPad<R>>(L)) * sizeof(cplx<R>);
+static size_t nl_ip_lds(int L, bool dfx = false) {
+  return ((size_t)L + (dfx ? 10 : 5) * (size_t)ip_pad<kIpPad<R>>(L)) * sizeof(cplx<R>);
 }
 
 int nonlinear_lds_bytes(const Plan& p, int precision, int /*dfx*/) {
@@ -993,7 +1001,7 @@ static int launch_nl_ip(const Plan& p, int B, const void* slabs, void* F, hipStr
 template <typename R, int NC>
 static int launch_nl_dfx_ip(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
                             hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
-  const size_t lds = 13 * (size_t)(64 * NC) * sizeof(cplx<R>);
+  const size_t lds = nl_ip_lds<R>(64 * NC, true);
   static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_nonlinear_dfx_ip<R, NC>, lds, cur)) return rc;
   hipLaunchKernelGGL((k_nonlinear_dfx_ip<R, NC>), dim3(p.nr, B), dim3(640), lds, s, p.tabM.shape,
@@ -1005,11 +1013,15 @@ template <typename R, bool DFX>
 static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slabs2, void* F,
                        hipStream_t s, const cplx<R>* tw, const cplx<R>* rot) {
   const size_t lds = 12 * (size_t)p.M * sizeof(cplx<R>);
+  // the in-place kernels take the per-pass twiddle layout
+  const cplx<R>* twp = std::is_same<R, float>::value
+                           ? reinterpret_cast<const cplx<R>*>(p.tabM.twp32)
+                           : reinterpret_cast<const cplx<R>*>(p.tabM.twp64);
   if (!DFX) {
     switch (nl_inplace_nc(p)) {
-      case 3: return launch_nl_ip<R, 3>(p, B, slabs, F, s, tw, rot);
-      case 6: return launch_nl_ip<R, 6>(p, B, slabs, F, s, tw, rot);
-      case 12: return launch_nl_ip<R, 12>(p, B, slabs, F, s, tw, rot);
+      case 3: return launch_nl_ip<R, 3>(p, B, slabs, F, s, twp, rot);
+      case 6: return launch_nl_ip<R, 6>(p, B, slabs, F, s, twp, rot);
+      case 12: return launch_nl_ip<R, 12>(p, B, slabs, F, s, twp, rot);
       default: break;
     }
     if (lds + 64 > 160 * 1024)
@@ -1020,10 +1032,10 @@ static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slab
                        rot, p.K, p.nr, (const R*)slabs, (R*)F);
     return hip_check(hipGetLastError(), "k_nonlinear_w");
   }
-  switch ((13 * (size_t)p.M * sizeof(cplx<R>) <= 160 * 1024) ? nl_inplace_nc(p) : 0) {
-    case 3: return launch_nl_dfx_ip<R, 3>(p, B, slabs, slabs2, F, s, tw, rot);
-    case 6: return launch_nl_dfx_ip<R, 6>(p, B, slabs, slabs2, F, s, tw, rot);
-    case 12: return launch_nl_dfx_ip<R, 12>(p, B, slabs, slabs2, F, s, tw, rot);
+  switch ((nl_ip_lds<R>(p.M, true) <= 160 * 1024) ? nl_inplace_nc(p) : 0) {
+    case 3: return launch_nl_dfx_ip<R, 3>(p, B, slabs, slabs2, F, s, twp, rot);
+    case 6: return launch_nl_dfx_ip<R, 6>(p, B, slabs, slabs2, F, s, twp, rot);
+    case 12: return launch_nl_dfx_ip<R, 12>(p, B, slabs, slabs2, F, s, twp, rot);
     default: break;
   }
   if (lds + 16 * 18 * sizeof(int) > 160 * 1024)
