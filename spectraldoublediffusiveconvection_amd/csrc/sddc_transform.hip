@@ -754,6 +754,7 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
   for (int c = 0; c < NC; ++c) O[ip_pad<PS>(lane + 64 * c)] = {o[c].x * s0, o[c].y * s1};
   wave_lds_sync();
   fft_run_ct<R, L, PS>(O, tw, false, lane);
+  asm volatile("" : "+v"(lane));   // recompute the projection's indices here
   const R i0 = (R)1 / s0, i1 = (R)1 / s1, invL = (R)1 / (R)L;
   R* f0 = F + ((size_t)(b * 4 + 2 * p) * nr + i) * K;
   R* f1 = F + ((size_t)(b * 4 + 2 * p + 1) * nr + i) * K;
@@ -802,7 +803,7 @@ constexpr int kIpPad = sizeof(R) == 4 ? 4 : 3;
 // once every product is in registers, wavefronts 0 and 1 reuse pair
 // buffers 3 and 4 for their forward transforms.
 template <typename R, int NC>
-__global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
+__global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
                                                       const cplx<R>* __restrict__ rot_g, int nr,
                                                       const R* __restrict__ slabs,
                                                       R* __restrict__ F) {
@@ -835,11 +836,13 @@ __global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
   R is[9];
 #pragma unroll
   for (int f = 0; f < 9; ++f) is[f] = sinv[f];
+  int ln = lane;
+  asm volatile("" : "+v"(ln));   // indices recomputed, not kept from the packing
   C o[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     if (p >= 2) break;
-    const int q = lane + 64 * c, qp = ip_pad<PS>(q);
+    const int q = ln + 64 * c, qp = ip_pad<PS>(q);
     const R gs = (R)grid_sign(q, L);
     const C w0 = G0[qp], w1 = G1[qp], w2 = G2[qp];
     const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
@@ -867,7 +870,7 @@ __global__ __launch_bounds__(320) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
 // LDS: twiddles (L), ten padded pair buffers; wavefronts 0 and 1 reuse pair
 // buffers 3 and 4 for their forward transforms once the products are formed.
 template <typename R, int NC>
-__global__ __launch_bounds__(640) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 5 : 1))) void k_nonlinear_dfx_ip(FftShape sh,
+__global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
                                                           const cplx<R>* __restrict__ tw_g,
                                                           const cplx<R>* __restrict__ rot_g,
                                                           int nr, const R* __restrict__ slabs,
@@ -902,11 +905,13 @@ __global__ __launch_bounds__(640) __attribute__((amdgpu_waves_per_eu(sizeof(R) =
     is[f] = sinv[f];
     id[f] = sinv[10 + f];
   }
+  int ln = lane;
+  asm volatile("" : "+v"(ln));   // indices recomputed, not kept from the packing
   C o[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     if (q >= 2) break;
-    const int qq = lane + 64 * c, qp = ip_pad<PS>(qq);
+    const int qq = ln + 64 * c, qp = ip_pad<PS>(qq);
     const R gs = (R)grid_sign(qq, L);
     const C w0 = buf[qp], w1 = buf[LP + qp], w2 = buf[2 * LP + qp];
     const C v0 = buf[5 * LP + qp], v1 = buf[6 * LP + qp], v2 = buf[7 * LP + qp];
