@@ -307,6 +307,26 @@ def test_batched_equals_alone_bitwise(c1):
         assert np.array_equal(both[b], alone)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spectral_instance_runs_same_bits(precision):
+    """The spectral stage walks runs of up to four instances per workgroup
+    (32 mode tiles at N_fm = 512: one instance at B = 18, two at B = 35, four
+    at B = 70, each with a partial last run), staging every instance's Ra_T
+    and Ra_S and prefetching the next one's state.  Every instance of the
+    batched step equals the same instance stepped alone, bit for bit."""
+    K, Nr = 512, 20
+    p = dev.Plan(ops.build(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0), precision=precision, max_batch=70)
+    rng = np.random.default_rng(21)
+    for B in (18, 35, 70):
+        X = 1e-3 * rng.standard_normal((B, 3 * K * (Nr - 1)))
+        Ra = np.linspace(2500.0, 9000.0, B)
+        Ras = np.linspace(100.0, 700.0, B)[::-1].copy()
+        both = H(p.step(T(X), Ra, Ras, nsteps=2))
+        for b in (0, 1, B // 2, B - 2, B - 1):
+            alone = H(p.step(T(X[b]), Ra[b], Ras[b], nsteps=2))[0]
+            assert np.array_equal(both[b], alone), (B, b)
+
+
 def test_graph_replay_matches_eager(c1):
     _, p = c1
     X0 = 1e-3 * np.random.default_rng(9).standard_normal(3 * C1["N_fm"] * (C1["N_r"] - 1))
