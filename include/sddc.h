@@ -73,9 +73,10 @@ typedef struct sddc_plan sddc_plan;
  *   solve_instances  instances per solve workgroup: 1, 2 or 4 (quad-lane
  *                    kernels), 3 (quad-lane, 2 per A4 workgroup and 4 per
  *                    NAB2 workgroup), 5 (as 3, the A4 chains with two
- *                    exchanges per row and one streamed block), 16 (MFMA
- *                    kernel).  1-4 give the same bits; 5 and 16 sum in
- *                    another order (equal to rounding), so the choice is
+ *                    exchanges per row and one streamed block), 6 (as 5,
+ *                    3 per A4 workgroup), 16 (MFMA kernel).  1-4 give the
+ *                    same bits, as do 5 and 6; 5, 6 and 16 sum in another
+ *                    order than 1-4 (equal to rounding), so the choice is
  *                    fixed per plan
  *   sub_batches      graphs only: the batch is split into this many parts
  *                    (1-8), each stepped on its own forked stream inside the

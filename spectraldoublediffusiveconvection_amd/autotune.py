@@ -37,8 +37,9 @@ def bucket(B):
 def candidates(N_fm, B=1):
     """Launch shapes tried for a class with N_fm modes (N_fm/2 rows per chain)
     at batch B: quad-lane solves with 1, 2 or 4 instances per workgroup or the
-    mixed widths 3 and 5 (2 per A4, 4 per NAB2 workgroup; 5 with the
-    two-exchange A4 chains), from B = 16 on the
+    mixed widths 3, 5 and 6 (2 per A4, 4 per NAB2 workgroup; 5 with the
+    two-exchange A4 chains, 6 with three of them per workgroup), from B = 16
+    on the
     16-instance MFMA solve, and from B = 64 on batches split in 2 or 4 parts
     on forked streams of the graph."""
     n = N_fm // 2
@@ -46,7 +47,7 @@ def candidates(N_fm, B=1):
     if B < 64:   # short segments only pay off while the device is not full
         segs.add(max(8, 4 * math.ceil(math.sqrt(n) / 4)))
     segs = sorted(segs)
-    nbs = (1, 2, 3, 4, 5, 16) if B >= 16 else (1, 2, 3, 4, 5)
+    nbs = (1, 2, 3, 4, 5, 6, 16) if B >= 16 else (1, 2, 3, 4, 5, 6)
     subs = (1, 2, 4) if B >= 64 else (1,)
     return [{"chain_segment": m, "solve_instances": nb, "sub_batches": sb}
             for m in segs for nb in nbs for sb in subs]

@@ -289,8 +289,8 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
   if (tuning && tuning->solve_instances != 0 && tuning->solve_instances != 1 &&
       tuning->solve_instances != 2 && tuning->solve_instances != 3 &&
       tuning->solve_instances != 4 && tuning->solve_instances != 5 &&
-      tuning->solve_instances != 16)
-    return bail(fail(SDDC_ERR_INVALID, "solve_instances must be 0, 1, 2, 3, 4, 5 or 16"));
+      tuning->solve_instances != 6 && tuning->solve_instances != 16)
+    return bail(fail(SDDC_ERR_INVALID, "solve_instances must be 0, 1, 2, 3, 4, 5, 6 or 16"));
   if (tuning && tuning->chain_segment < 0)
     return bail(fail(SDDC_ERR_INVALID, "chain_segment must be >= 0"));
   if (tuning && (tuning->sub_batches < 0 || tuning->sub_batches > 8))

@@ -429,9 +429,12 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs 
 // workgroups past the end of the batch for their width return at once.
 //   solve_instances 3: NBA 2, NBN 4 (one exchange per A4 row)
 //   solve_instances 5: NBA 2, NBN 4, X2
-// Measured at N_fm = 512, N_r = 40 (B = 64 and 256): 5 is the fastest in
-// the sub-batched graph (it streams a third of 3's A4 bytes); four A4 or
-// eight NAB2 instances per workgroup were slower than both.
+//   solve_instances 6: NBA 3, NBN 4, X2
+// Measured at N_fm = 512, N_r = 40, B = 256: the full-batch solve takes
+// 374 µs (3), 362 µs (5) and 303 µs (6); the A4 chains stream a third (5)
+// and two ninths (6) of 3's bytes per instance.  Four A4 or eight NAB2
+// instances per workgroup were slower (more per-row work on three
+// wavefronts), as was three NAB2 instances.
 template <int NRS, int NBA, int NBN, bool X2>
 __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(ChainArgs a) {
   using S = CShape<NRS>;
@@ -942,6 +945,7 @@ static int dispatch_nb(const ChainArgs& a, int NB, int B, hipStream_t s) {
     case 2: return launch_chain_t<NRS, 2>(a, B, s);
     case 3: return launch_chain_mixed_t<NRS, 2, 4, false>(a, B, s);
     case 5: return launch_chain_mixed_t<NRS, 2, 4, true>(a, B, s);
+    case 6: return launch_chain_mixed_t<NRS, 3, 4, true>(a, B, s);
     case 4: return launch_chain_t<NRS, 4>(a, B, s);
     default: return fail(SDDC_ERR_INVALID, "unsupported instances per workgroup");
   }

@@ -151,7 +151,7 @@ def test_solve_segmentations(K, Nr, max_batch, sym):
 @pytest.mark.parametrize("K,Nr", [(64, 20), (512, 40), (10, 9), (32, 65)])
 @pytest.mark.parametrize("segment", [0, 8])
 @pytest.mark.parametrize("sym", [False, True])
-@pytest.mark.parametrize("width", [16, 3, 5])
+@pytest.mark.parametrize("width", [16, 3, 5, 6])
 def test_solve_mfma(K, Nr, segment, sym, width):
     """The 16-instance MFMA chain solve (solve_instances = 16) and the
     mixed-width quad solve (3: 2 instances per A4 workgroup, 4 per NAB2
