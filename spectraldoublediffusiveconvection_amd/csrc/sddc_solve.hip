@@ -145,10 +145,14 @@ __device__ __forceinline__ Inst<NB> instances(const ChainArgs& a, int field, int
     const int bc = valid ? b : a.B - 1;
     in.g[t] = a.G + (size_t)bc * a.gstride + a.goff[field];
     // instances past the batch write into scratch, so every wavefront runs
-    // the same instructions
+    // the same instructions.  Their segment exit states go to those of the
+    // last instance, whose inputs they read and whose values they repeat
+    // bit for bit: the exit-state workspace holds the batch's instances
+    // only, and with three instances per workgroup the padded instances
+    // would reach past it (or, in a sub-batch, into the next sub-batch's).
     in.x[t] = valid ? a.X + (size_t)b * a.xstride + a.xoff[field] : a.scratch;
     in.p[t] = (a.P && valid) ? a.P + (size_t)b * N : a.scratch + N;
-    in.ex[t] = a.EX + (((size_t)b * 3 + field) * 2 + chain) * exs;
+    in.ex[t] = a.EX + (((size_t)bc * 3 + field) * 2 + chain) * exs;
   }
   return in;
 }

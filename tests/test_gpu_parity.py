@@ -186,6 +186,56 @@ def test_solve_mfma(K, Nr, segment, sym, width):
         assert rel(Y[b], ref) < 1e-12, b
 
 
+@pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 16])
+def test_solve_widths_padded_instances(width):
+    """Regression test: batches that do not fill the last workgroup of a
+    width (B = 4 and 5; with three instances per A4 workgroup, instances 4
+    and 5 of B = 4 are padding) and sub-batched graphs whose parts do not
+    divide by the width, in segments of 8 rows (so the joins read the
+    segment exit states).  Padded instances write the exit states of the
+    last real one; before, they wrote past the exit-state workspace or into
+    the next sub-batch's."""
+    K, Nr, d, dt, Pr, Tau = 64, 20, 0.35, 5e-3, 1.0, 1 / 15.0
+    o = O.build_operators(K, Nr, d, dt, Pr, Tau)
+    op = ops.build(K, Nr, d, dt, Pr, Tau)
+    nr = Nr - 1
+    rng = np.random.default_rng(width)
+    dts = (Pr * dt, dt, Tau * dt)
+    for B in (4, 5):
+        p = dev.Plan(op, max_batch=B, tuning={"chain_segment": 8, "solve_instances": width})
+        g = rng.standard_normal((B, K * nr))
+        for field in range(3):
+            f = H(p.bsub(T(g), field))
+            for b in range(B):
+                if field == 0:
+                    ref = O.a4_bsub(g[b], o.Linv_A4, o.D2_A4, o.IR4_A4, o.IR2_A4, K, nr, dts[0],
+                                    False)
+                else:
+                    ref = O.nab2_bsub(g[b], o.Linv_T if field == 1 else o.Linv_S, K, nr,
+                                      dts[field], False)
+                assert rel(f[b], ref) < 1e-14, (B, field, b)
+        p.close()
+    B = 64
+    p = dev.Plan(op, max_batch=B, tuning={"chain_segment": 8, "solve_instances": width,
+                                          "sub_batches": 2})
+    X = 1e-3 * rng.standard_normal((B, 3 * K * nr))
+    Ra = np.linspace(2500.0, 9000.0, B)
+    eager = T(X)
+    xg = T(X).clone()
+    gr = p.graph(xg, Ra, 500.0, nsteps=3)   # two parts of 32 on forked streams
+    for rep in range(2):
+        gr.launch()
+        eager = p.step(eager, Ra, 500.0, nsteps=3)
+        assert np.array_equal(H(xg), H(eager)), rep
+    gr.close()
+    for b in (0, 31, 32, 63):
+        ref = X[b]
+        for _ in range(3):
+            ref = O.step(ref, o, Ra[b], 500.0)
+        assert rel(H(p.step(T(X[b:b + 1]), Ra[b], 500.0, nsteps=3))[0], ref) < 1e-12, b
+    p.close()
+
+
 def test_single_step_and_linearised(c1):
     o, p = c1
     for name in ("X_big", "X_small"):
