@@ -472,7 +472,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
     }
     auto* xa = reinterpret_cast<double (*)[NBA][NRS + 64]>(xs);
     if constexpr (X2)
-      local_a4x<NRS, NBA>(a, start, k0, k1, in, xa, exit_seg);
+      local_a4x<NRS, NBA, kDepthMixed>(a, start, k0, k1, in, xa, exit_seg);
     else
       local_a4<NRS, NBA, kDepthMixed>(a, start, k0, k1, in, xa, exit_seg);
     return;
