@@ -1077,12 +1077,6 @@ static void pack_state(const double* A, int nr, int NRS, int NRP, double* out) {
   pack_quad(wide.data(), nr, 2 * NRS, 2 * NRS, NRP, 2 * NRS, out);
 }
 
-// Rows per segment for chains of n rows.  Small batches are latency bound
-// (depth m + n/m, smallest near m = sqrt(n)); mid-size batches trade depth for
-// fewer corrections (m = 2 sqrt(n)); large batches fill the device without
-// segments, which avoid the corrections' extra work (measured at n = 256 on
-// MI355X).  SDDC_CHAIN_SEGMENT overrides.  Fixed per plan, so that the bits
-// of a result do not depend on the batch it was computed in.
 // MFMA packing of an nrows x ncols row-major matrix into an NRP x NC tile:
 // element (16w + l%16, 4ks + l/16) at ((w*(NC/8) + ks/2)*64 + l)*2 + ks%2, so
 // that lane l of wavefront w loads its A fragments of k steps 2kp, 2kp+1 with
@@ -1100,6 +1094,12 @@ static void pack_mfma(const double* A, int nrows, int ncols, int lda, int NRP, i
         }
 }
 
+// Rows per segment for chains of n rows.  Small batches are latency bound
+// (depth m + n/m, smallest near m = sqrt(n)); mid-size batches trade depth for
+// fewer corrections (m = 2 sqrt(n)); large batches fill the device without
+// segments, which avoid the corrections' extra work (measured at n = 256 on
+// MI355X).  SDDC_CHAIN_SEGMENT overrides.  Fixed per plan, so that the bits
+// of a result do not depend on the batch it was computed in.
 static int default_segment(int n, int max_batch) {
   if (const char* env = std::getenv("SDDC_CHAIN_SEGMENT")) {
     const int m = std::atoi(env);
