@@ -408,15 +408,6 @@ __device__ cplx<R>* fft_run_w(cplx<R>* a, cplx<R>* b, const FftShape& sh,
 // and the buffer is not restrict-qualified, so the compiler keeps every read
 // of a pass ahead of its writes.  ip_shape_ok checks the chunk condition.
 // ---------------------------------------------------------------------------
-// Padded LDS index of element x of an in-place transform buffer: one empty
-// slot after every 2^PS elements (PS = 0: no padding).  With 2^PS elements
-// per 128 bytes, the strided writes of the early Stockham passes (stride
-// RADIX * Ns elements) fall in different banks.
-template <int PS>
-__host__ __device__ constexpr int ip_pad(int x) {
-  return PS ? x + (x >> PS) : x;
-}
-
 template <int RADIX>
 struct IpChunk {
   static constexpr int U = RADIX >= 5 ? 2 : (RADIX == 4 ? 3 : 4);
@@ -436,7 +427,7 @@ inline bool ip_shape_ok(const FftShape& sh) {
   return true;
 }
 
-template <typename R, int RADIX, int PS>
+template <typename R, int RADIX>
 __device__ __forceinline__ void fft_pass_ip(cplx<R>* buf, int L, int Ns,
                                             const cplx<R>* __restrict__ tw, bool inv, int lane) {
   constexpr int U = IpChunk<RADIX>::U;
@@ -453,7 +444,7 @@ __device__ __forceinline__ void fft_pass_ip(cplx<R>* buf, int L, int Ns,
     const int k = j - jq * Ns;
     base[u] = jq * Ns * RADIX + k;
 #pragma unroll
-    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[ip_pad<PS>(j + s * nb)];
+    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[j + s * nb];
 #pragma unroll
     for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, s * k * st, inv);
   }
@@ -470,25 +461,25 @@ __device__ __forceinline__ void fft_pass_ip(cplx<R>* buf, int L, int Ns,
   for (int u = 0; u < U; ++u)
     if (lane + 64 * u < nb)
 #pragma unroll
-      for (int s = 0; s < RADIX; ++s) buf[ip_pad<PS>(base[u] + s * Ns)] = v[u][s];
+      for (int s = 0; s < RADIX; ++s) buf[base[u] + s * Ns] = v[u][s];
 }
 
 // All passes of one transform in place by one wavefront (shape checked by
-// ip_shape_ok on the host); the buffer is indexed through ip_pad<PS>.
-template <typename R, int PS = 0>
+// ip_shape_ok on the host).
+template <typename R>
 __device__ void fft_run_ip(cplx<R>* buf, const FftShape& sh, const cplx<R>* __restrict__ tw,
                            bool inv, int lane) {
   int Ns = 1;
   for (int p = 0; p < sh.npass; ++p) {
     const int r = sh.radix[p];
     if (r == 4)
-      fft_pass_ip<R, 4, PS>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 4>(buf, sh.L, Ns, tw, inv, lane);
     else if (r == 2)
-      fft_pass_ip<R, 2, PS>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 2>(buf, sh.L, Ns, tw, inv, lane);
     else if (r == 3)
-      fft_pass_ip<R, 3, PS>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 3>(buf, sh.L, Ns, tw, inv, lane);
     else
-      fft_pass_ip<R, 5, PS>(buf, sh.L, Ns, tw, inv, lane);
+      fft_pass_ip<R, 5>(buf, sh.L, Ns, tw, inv, lane);
     wave_lds_sync();
     Ns *= r;
   }
@@ -531,7 +522,7 @@ __host__ __device__ constexpr int ip_twiddle_offset(int L, int P) {
   return off;
 }
 
-template <typename R, int L, int RADIX, int NS, int PS, int TWO>
+template <typename R, int L, int RADIX, int NS, int TWO>
 __device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restrict__ tw,
                                             bool inv, int lane) {
   constexpr int NB = L / RADIX;                 // butterflies
@@ -547,7 +538,7 @@ __device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restr
     const int k = j - jq * NS;
     base[u] = jq * NS * RADIX + k;
 #pragma unroll
-    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[ip_pad<PS>(j + s * NB)];
+    for (int s = 0; s < RADIX; ++s) v[u][s] = buf[j + s * NB];
 #pragma unroll
     for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, TWO + (s - 1) * NS + k, inv);
   }
@@ -564,18 +555,18 @@ __device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restr
   for (int u = 0; u < U; ++u)
     if (NB % 64 == 0 || lane + 64 * u < NB)
 #pragma unroll
-      for (int s = 0; s < RADIX; ++s) buf[ip_pad<PS>(base[u] + s * NS)] = v[u][s];
+      for (int s = 0; s < RADIX; ++s) buf[base[u] + s * NS] = v[u][s];
 }
 
 // All passes; tw is the per-pass twiddle table (ip_twiddle_offset).
-template <typename R, int L, int PS, int P = 0, int NS = 1>
+template <typename R, int L, int P = 0, int NS = 1>
 __device__ __forceinline__ void fft_run_ct(cplx<R>* buf, const cplx<R>* __restrict__ tw, bool inv,
                                            int lane) {
   if constexpr (P < ip_npass(L)) {
     constexpr int RADIX = ip_radix(L, P);
-    fft_pass_ct<R, L, RADIX, NS, PS, ip_twiddle_offset(L, P)>(buf, tw, inv, lane);
+    fft_pass_ct<R, L, RADIX, NS, ip_twiddle_offset(L, P)>(buf, tw, inv, lane);
     wave_lds_sync();
-    fft_run_ct<R, L, PS, P + 1, NS * RADIX>(buf, tw, inv, lane);
+    fft_run_ct<R, L, P + 1, NS * RADIX>(buf, tw, inv, lane);
   }
 }
 

@@ -611,7 +611,7 @@ __global__ __launch_bounds__(320) void k_nonlinear_w(FftShape sh, const cplx<R>*
 // changes that cut LDS from 12 L to 8 L complex values for F(X, X) (three
 // workgroups per CU at L = 768 in float):
 //  * each transform runs in place in one buffer (fft_run_ct: the pass
-//    schedule fixed at compile time, the buffer padded against bank conflicts);
+//    schedule fixed at compile time);
 //  * the slab rows are not staged in LDS: lane l holds modes l + 64j in
 //    registers, the mirrored coefficient at index L - k needed by the
 //    Makhoul packing comes from lane (64 - l) mod 64 by a cross-lane read;
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(320) void k_nonlinear_w(FftShape sh, const cplx<R>*
 // Loads field pair p of a slab set (one wavefront), scales both series by a
 // power of two and writes their packed, rotated spectrum W_k into A.  sinv
 // receives the two unscaling factors.
-template <typename R, int NC, int PS>
+template <typename R, int NC>
 __device__ __forceinline__ void ip_pack_pair(const SlabRows<R>& sg, int p, cplx<R>* A,
                                              const cplx<R>* __restrict__ rot_g, R* sinv,
                                              int lane) {
@@ -699,7 +699,7 @@ __device__ __forceinline__ void ip_pack_pair(const SlabRows<R>& sg, int p, cplx<
     const R au = us ? mu[c] : dcu, av = vs ? mv[c] : dcv;
     const R aum = k ? (us ? dcu : mu[c]) : (R)0;
     const R avm = k ? (vs ? dcv : mv[c]) : (R)0;
-    A[ip_pad<PS>(k)] = cmul(C{au + avm, av - aum}, rr[c]);
+    A[k] = cmul(C{au + avm, av - aum}, rr[c]);
   }
 }
 
@@ -726,7 +726,7 @@ struct IpTwiddles {
 // The second phase for wavefront p in {0, 1}: products o (registers) scaled
 // to unit magnitude, written to O, forward FFT in place, projection onto the
 // K modes and stores of output rows 2p, 2p+1.
-template <typename R, int NC, int PS>
+template <typename R, int NC>
 __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O, int p,
                                                    const cplx<R>* tw, const FftShape& sh,
                                                    const cplx<R>* __restrict__ rot_g,
@@ -751,9 +751,9 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
   e1 = __ockl_wfred_max_i32(e1);
   const R s0 = pow2_scale_e<R>(e0), s1 = pow2_scale_e<R>(e1);
 #pragma unroll
-  for (int c = 0; c < NC; ++c) O[ip_pad<PS>(lane + 64 * c)] = {o[c].x * s0, o[c].y * s1};
+  for (int c = 0; c < NC; ++c) O[lane + 64 * c] = {o[c].x * s0, o[c].y * s1};
   wave_lds_sync();
-  fft_run_ct<R, L, PS>(O, tw, false, lane);
+  fft_run_ct<R, L>(O, tw, false, lane);
   asm volatile("" : "+v"(lane));   // recompute the projection's indices here
   const R i0 = (R)1 / s0, i1 = (R)1 / s1, invL = (R)1 / (R)L;
   R* f0 = F + ((size_t)(b * 4 + 2 * p) * nr + i) * K;
@@ -765,8 +765,8 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
     const int ka = (p == 0) ? L - k : k;
     const int kam = ka < L ? ka : ka - L;
     const int kb = (L - ka) < L ? L - ka : L - ka - L;
-    za[j] = O[ip_pad<PS>(kam)];
-    zb[j] = O[ip_pad<PS>(kb)];
+    za[j] = O[kam];
+    zb[j] = O[kb];
   }
 #pragma unroll
   for (int j = 0; j < NK; ++j) {
@@ -793,13 +793,8 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
   }
 }
 
-// Bank padding of the F(X, X) kernel's transform buffers: one slot per
-// 128 bytes of complex values.
-template <typename R>
-constexpr int kIpPad = sizeof(R) == 4 ? 4 : 3;
-
 // F(X, X): five wavefronts, wavefront p synthesises field pair p.
-// LDS: twiddles (L) and five pair buffers of LP (= L padded) values each;
+// LDS: twiddles (L) and five pair buffers of L values each;
 // once every product is in registers, wavefronts 0 and 1 reuse pair
 // buffers 3 and 4 for their forward transforms.
 template <typename R, int NC>
@@ -808,7 +803,6 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
                                                       const R* __restrict__ slabs,
                                                       R* __restrict__ F) {
   constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
-  constexpr int PS = kIpPad<R>, LP = ip_pad<PS>(L);
   static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ R sinv[10];   // unscaling factors of the nine fields
@@ -820,19 +814,19 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
   IpTwiddles<R, NC, 320> twr;
   twr.load(tw_g);
-  C* A = buf + (size_t)p * LP;
-  ip_pack_pair<R, NC, PS>(sg, p, A, rot_g, sinv + 2 * p, lane);
+  C* A = buf + (size_t)p * L;
+  ip_pack_pair<R, NC>(sg, p, A, rot_g, sinv + 2 * p, lane);
   twr.store(tw);
   __syncthreads();   // twiddles in LDS
-  fft_run_ct<R, L, PS>(A, tw, true, lane);
+  fft_run_ct<R, L>(A, tw, true, lane);
   __syncthreads();
 
   // products of the grid fields (Matrix_Operators.py:791-793)
   const C* G0 = buf;
-  const C* G1 = buf + LP;
-  const C* G2 = buf + 2 * LP;
-  const C* G3 = buf + 3 * LP;
-  const C* G4 = buf + 4 * LP;
+  const C* G1 = buf + L;
+  const C* G2 = buf + 2 * L;
+  const C* G3 = buf + 3 * L;
+  const C* G4 = buf + 4 * L;
   R is[9];
 #pragma unroll
   for (int f = 0; f < 9; ++f) is[f] = sinv[f];
@@ -842,9 +836,9 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     if (p >= 2) break;
-    const int q = ln + 64 * c, qp = ip_pad<PS>(q);
+    const int q = ln + 64 * c;
     const R gs = (R)grid_sign(q, L);
-    const C w0 = G0[qp], w1 = G1[qp], w2 = G2[qp];
+    const C w0 = G0[q], w1 = G1[q], w2 = G2[q];
     const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
     const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
     if (p == 0) {
@@ -853,7 +847,7 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
       o[c] = {gs * (JT * Om), gs * (kDpsi * Om + Dpsi * kOm)};
     } else {
       // Matrix_Operators.py:792-793
-      const C w3 = G3[qp], w4 = G4[qp];
+      const C w3 = G3[q], w4 = G4[q];
       const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
               DC = w4.x * is[8];
       o[c] = {JT * DT - Dpsi * kT, JT * DC - Dpsi * kC};
@@ -861,13 +855,13 @@ __global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>
   }
   __syncthreads();   // every read of the pair buffers is done
   if (p >= 2) return;
-  ip_forward_project<R, NC, PS>(o, buf + (size_t)(3 + p) * LP, p, tw, sh, rot_g, F, b, i, nr,
+  ip_forward_project<R, NC>(o, buf + (size_t)(3 + p) * L, p, tw, sh, rot_g, F, b, i, nr,
                                 lane);
 }
 
 // F(X, dv) + F(dv, X) (NLIN_DFX, Matrix_Operators.py:807-898): ten
 // wavefronts, wavefront q < 5 synthesises pair q of X, q >= 5 pair q-5 of dv.
-// LDS: twiddles (L), ten padded pair buffers; wavefronts 0 and 1 reuse pair
+// LDS: twiddles (L), ten pair buffers; wavefronts 0 and 1 reuse pair
 // buffers 3 and 4 for their forward transforms once the products are formed.
 template <typename R, int NC>
 __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
@@ -877,7 +871,6 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
                                                           const R* __restrict__ slabs2,
                                                           R* __restrict__ F) {
   constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
-  constexpr int PS = kIpPad<R>, LP = ip_pad<PS>(L);
   static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ R sinv[20];   // unscaling factors: fields of X, then of dv
@@ -890,11 +883,11 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
   const SlabRows<R> sg = slab_rows<R>(q < 5 ? slabs : slabs2, b, i, nr, K);
   IpTwiddles<R, NC, 640> twr;
   twr.load(tw_g);
-  C* A = buf + (size_t)q * LP;
-  ip_pack_pair<R, NC, PS>(sg, p, A, rot_g, sinv + (q < 5 ? 0 : 10) + 2 * p, lane);
+  C* A = buf + (size_t)q * L;
+  ip_pack_pair<R, NC>(sg, p, A, rot_g, sinv + (q < 5 ? 0 : 10) + 2 * p, lane);
   twr.store(tw);
   __syncthreads();   // twiddles in LDS
-  fft_run_ct<R, L, PS>(A, tw, true, lane);
+  fft_run_ct<R, L>(A, tw, true, lane);
   __syncthreads();
 
   // Frechet derivative (Matrix_Operators.py:884-887): every term is (field of
@@ -911,10 +904,10 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     if (q >= 2) break;
-    const int qq = ln + 64 * c, qp = ip_pad<PS>(qq);
+    const int qq = ln + 64 * c;
     const R gs = (R)grid_sign(qq, L);
-    const C w0 = buf[qp], w1 = buf[LP + qp], w2 = buf[2 * LP + qp];
-    const C v0 = buf[5 * LP + qp], v1 = buf[6 * LP + qp], v2 = buf[7 * LP + qp];
+    const C w0 = buf[qq], w1 = buf[L + qq], w2 = buf[2 * L + qq];
+    const C v0 = buf[5 * L + qq], v1 = buf[6 * L + qq], v2 = buf[7 * L + qq];
     const R JT = w0.x * is[0], Om = gs * w0.y * is[1];
     const R kOm = w1.x * is[2], Dpsi = gs * w1.y * is[3];
     const R dJT = v0.x * id[0], dOm = gs * v0.y * id[1];
@@ -928,8 +921,8 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
       zy += dkDpsi * Om;
       o[c] = {gs * zx, gs * zy};
     } else {
-      const C w3 = buf[3 * LP + qp], w4 = buf[4 * LP + qp];
-      const C v3 = buf[8 * LP + qp], v4 = buf[9 * LP + qp];
+      const C w3 = buf[3 * L + qq], w4 = buf[4 * L + qq];
+      const C v3 = buf[8 * L + qq], v4 = buf[9 * L + qq];
       const R kT = gs * w2.y * is[5], DT = w3.x * is[6], kC = gs * w3.y * is[7],
               DC = w4.x * is[8];
       const R dkT = gs * v2.y * id[5], dDT = v3.x * id[6], dkC = gs * v3.y * id[7],
@@ -946,7 +939,7 @@ __global__ __launch_bounds__(640) void k_nonlinear_dfx_ip(FftShape sh,
   }
   __syncthreads();   // every read of the pair buffers is done
   if (q >= 2) return;
-  ip_forward_project<R, NC, PS>(o, buf + (size_t)(3 + q) * LP, q, tw, sh, rot_g, F, b, i, nr,
+  ip_forward_project<R, NC>(o, buf + (size_t)(3 + q) * L, q, tw, sh, rot_g, F, b, i, nr,
                                 lane);
 }
 
@@ -976,16 +969,16 @@ int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& curren
   return SDDC_OK;
 }
 
-// Dynamic LDS of the in-place kernels: twiddles (L) and the padded pair
+// Dynamic LDS of the in-place kernels: twiddles (L) and the pair
 // buffers (five for F(X, X), ten for the Frechet derivative).
 template <typename R>
 static size_t nl_ip_lds(int L, bool dfx = false) {
-  return ((size_t)L + (dfx ? 10 : 5) * (size_t)ip_pad<kIpPad<R>>(L)) * sizeof(cplx<R>);
+  return ((size_t)L + (dfx ? 10 : 5) * (size_t)L) * sizeof(cplx<R>);
 }
 
 int nonlinear_lds_bytes(const Plan& p, int precision, int /*dfx*/) {
   const size_t c = precision == SDDC_MIXED ? sizeof(float2) : sizeof(double2);
-  // in place: twiddles + 5 pair + 2 product buffers (padded); else 2 x 5
+  // in place: twiddles + 5 pair buffers (the products reuse two); else 2 x 5
   // working arrays + twiddles + rotations
   if (nl_inplace_nc(p))
     return (int)(precision == SDDC_MIXED ? nl_ip_lds<float>(p.M) : nl_ip_lds<double>(p.M));
