@@ -170,10 +170,14 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
   const int r = 16 * w + (lane >> 2), kq = lane & 3;
   const bool rv = r < nr;
   const int rr = rv ? r : nr - 1;
+  // lanes of padding rows read the operator entries of row nr-1 (same cache
+  // lines as the valid lanes, so no extra memory traffic); their results
+  // only meet zero operator columns and are never stored
+  const int lq = lane - 4 * (r - rr);
   const bool writer = kq == 0 && r < NRS;
   const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
   const double dtf = a.dtf[field];
-  const double* lb = a.Lq[field] + w * S::NQ * 128 + 2 * lane;
+  const double* lb = a.Lq[field] + w * S::NQ * 128 + 2 * lq;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
   double2 blk[KD][S::NQ];
   double gv[KD][NB];
@@ -232,13 +236,17 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
   const int r = 16 * w + (lane >> 2), kq = lane & 3;
   const bool rv = r < nr;
   const int rr = rv ? r : nr - 1;
+  // lanes of padding rows read the operator entries of row nr-1 (same cache
+  // lines as the valid lanes, so no extra memory traffic); their results
+  // only meet zero operator columns and are never stored
+  const int lq = lane - 4 * (r - rr);
   const bool writer = kq == 0 && r < NRS;
   const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
   const double dta = a.dtf[0];
   const double ir4 = a.vecs[V_IR4A4 * S::NRP + rr];
   const double ir2 = a.vecs[V_IR2A4 * S::NRP + rr];
-  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lane;
-  const double* mb = a.DLq + w * S::NQ * 128 + 2 * lane;
+  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lq;
+  const double* mb = a.DLq + w * S::NQ * 128 + 2 * lq;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
   double2 blk[KD][S::NQ], mblk[KD][S::NQ];
   double gv[KD][NB];
@@ -317,14 +325,18 @@ __device__ __forceinline__ void local_a4x(const ChainArgs& a, int start, int k0,
   const int r = 16 * w + (lane >> 2), kq = lane & 3;
   const bool rv = r < nr;
   const int rr = rv ? r : nr - 1;
+  // lanes of padding rows read the operator entries of row nr-1 (same cache
+  // lines as the valid lanes, so no extra memory traffic); their results
+  // only meet zero operator columns and are never stored
+  const int lq = lane - 4 * (r - rr);
   const bool writer = kq == 0 && r < NRS;
   const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
   const double dta = a.dtf[0];
   const double ir4 = a.vecs[V_IR4A4 * S::NRP + rr];
   const double ir2 = a.vecs[V_IR2A4 * S::NRP + rr];
   double2 d2[S::NQ];
-  load_blk<S::NQ>(d2, a.D2q + w * S::NQ * 128 + 2 * lane);
-  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lane;
+  load_blk<S::NQ>(d2, a.D2q + w * S::NQ * 128 + 2 * lq);
+  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lq;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
   double2 blk[KD][S::NQ];
   double gv[KD][NB];
@@ -506,11 +518,15 @@ __device__ __forceinline__ void join_nab2(const ChainArgs& a, int field, int cha
   const int r = 16 * w + (lane >> 2), kq = lane & 3;
   const bool rv = r < nr;
   const int rr = rv ? r : nr - 1;
+  // lanes of padding rows read the operator entries of row nr-1 (same cache
+  // lines as the valid lanes, so no extra memory traffic); their results
+  // only meet zero operator columns and are never stored
+  const int lq = lane - 4 * (r - rr);
   const bool writer = kq == 0 && r < NRS;
   const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
   // entry state: s_0 = 0, s_{u+1} = ex_u + E_u s_u for u < seg
-  const double* eb = a.Eq[field] + (size_t)chain * a.nseg * S::BLK + w * S::NQ * 128 + 2 * lane;
+  const double* eb = a.Eq[field] + (size_t)chain * a.nseg * S::BLK + w * S::NQ * 128 + 2 * lq;
   double2 e[kDepthE][S::NQ];
   double exv[kDepthE][NB];
   auto fetch_e = [&](int u, double2 (&eq)[S::NQ], double (&x)[NB]) {
@@ -553,7 +569,7 @@ __device__ __forceinline__ void join_nab2(const ChainArgs& a, int field, int cha
 #pragma unroll
     for (int q = 0; q < 2 * S::NQ; ++q) xl[t][q] = xs[0][t][kq * S::CPL + q];
   // corrections f_row += Q_row s (independent rows)
-  const double* qb = a.Qq[field] + w * S::NQ * 128 + 2 * lane;
+  const double* qb = a.Qq[field] + w * S::NQ * 128 + 2 * lq;
   double2 qv[kDepth][S::NQ];
   double fl[kDepth][NB];
   auto fetch_q = [&](int k, double2 (&qq)[S::NQ], double (&f)[NB]) {
@@ -595,13 +611,17 @@ __device__ __forceinline__ void join_a4(const ChainArgs& a, int chain, int start
   const int r = 16 * w + (lane >> 2), kq = lane & 3;
   const bool rv = r < nr;
   const int rr = rv ? r : nr - 1;
+  // lanes of padding rows read the operator entries of row nr-1 (same cache
+  // lines as the valid lanes, so no extra memory traffic); their results
+  // only meet zero operator columns and are never stored
+  const int lq = lane - 4 * (r - rr);
   const bool writer = kq == 0 && r < NRS;
   const int wpf = writer ? vpos(r, CPL2) : 2 * NRS + lane;
   const int wpb = writer ? vpos(NRS + r, CPL2) : 2 * NRS + lane;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
   // state (f_e, bf_e): column c < NRS is f_e[c], column NRS + c is bf_e[c];
   // E is stored as its f_e rows then its bf_e rows.
-  const double* eb = a.Eq[0] + (size_t)chain * a.nseg * 2 * BLK2 + w * NQ2 * 128 + 2 * lane;
+  const double* eb = a.Eq[0] + (size_t)chain * a.nseg * 2 * BLK2 + w * NQ2 * 128 + 2 * lq;
   double2 ef[kDepthE][NQ2], ebv[kDepthE][NQ2];
   double exf[kDepthE][NB], exb[kDepthE][NB];
   auto fetch_e = [&](int u, double2 (&qf)[NQ2], double2 (&qb)[NQ2], double (&xf)[NB],
@@ -657,7 +677,7 @@ __device__ __forceinline__ void join_a4(const ChainArgs& a, int chain, int start
 #pragma unroll
     for (int q = 0; q < CPL2; ++q) xl[t][q] = xs[0][t][kq * CPL2 + q];
   // corrections f_row += Q_row s, and P_row = f_e of the row
-  const double* qb = a.Qq[0] + w * NQ2 * 128 + 2 * lane;
+  const double* qb = a.Qq[0] + w * NQ2 * 128 + 2 * lq;
   double2 qv[kDepth][NQ2];
   double fl[kDepth][NB];
   auto fetch_q = [&](int k, double2 (&qq)[NQ2], double (&f)[NB]) {
