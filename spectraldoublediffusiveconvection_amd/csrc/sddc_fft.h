@@ -539,13 +539,16 @@ __device__ __forceinline__ void fft_pass_ct(cplx<R>* buf, const cplx<R>* __restr
     base[u] = jq * NS * RADIX + k;
 #pragma unroll
     for (int s = 0; s < RADIX; ++s) v[u][s] = buf[j + s * NB];
+    // the first pass (NS = 1) has only the twiddle W^0 = 1: none is read
+    if constexpr (NS > 1)
 #pragma unroll
-    for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, TWO + (s - 1) * NS + k, inv);
+      for (int s = 1; s < RADIX; ++s) w[u][s] = tw_get(tw, TWO + (s - 1) * NS + k, inv);
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
+    if constexpr (NS > 1)
 #pragma unroll
-    for (int s = 1; s < RADIX; ++s) v[u][s] = cmul(v[u][s], w[u][s]);
+      for (int s = 1; s < RADIX; ++s) v[u][s] = cmul(v[u][s], w[u][s]);
     if (RADIX == 2) dft2(v[u]);
     if (RADIX == 3) dft3(v[u], inv);
     if (RADIX == 4) dft4(v[u], inv);
