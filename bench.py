@@ -6,7 +6,8 @@ A "step" is one batched step of B instances of the size class N_r=40,
 N_fm=512 (d=0.35, Pr=1, Tau=1/15, dt=5e-3) in mixed precision (float32
 transforms, products and Chebyshev GEMMs; float64 linear terms and solves),
 replayed from a HIP graph.  Ra_T is spread over [2500, 3000] and Ra_s is 500
-(the range over which the run stays bounded).  States are
+(at Ra_T = 9000 the conductive state grows like exp(~20 t) and overflows
+within a few hundred steps; over [2500, 3000] the run stays bounded).  States are
 seeded normal vectors scaled to ||X||_2 = 1e-3, resident in HBM before the
 timed region.  ``value`` is the aggregate instance-steps per second.
 

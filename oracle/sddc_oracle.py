@@ -1,8 +1,8 @@
 """CPU oracle for the pseudospectral double-diffusive time step.
 
-TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
-checker or as the timed CPU baseline.  The product package
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, the ``smoke()`` check of the
+repository's entry module and the ``cpu_baseline`` leg of ``bench.py`` may
+import this module, and only as the checker or as the timed CPU baseline.  The product package
 (``spectraldoublediffusiveconvection_amd``) never imports it.
 
 It restates, in NumPy/SciPy float64, the algorithm of the reference
