@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=256)
-    ap.add_argument("--graph-steps", type=int, default=10)
+    ap.add_argument("--graph-steps", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
@@ -69,7 +69,7 @@ def measured_traffic(kernel, B):
     return rec["traffic_bytes"]
 
 
-def sweep_c4(tuning_table, n=256, graph_steps=10, launches=5):
+def sweep_c4(tuning_table, n=256, graph_steps=40, launches=2):
     """Mixed-resolution sweep: n instances with N_r in {20,30,40} and N_fm in
     {64,128,256,512} (seeded), one HIP graph over all size classes."""
     import torch
