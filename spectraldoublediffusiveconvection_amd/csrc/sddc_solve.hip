@@ -26,9 +26,10 @@
 // quad-packed, so that every 16-byte load instruction of a wavefront reads
 // 1 KiB of consecutive memory: element (r, c) of an NRP x NC matrix sits at
 //   ((w*(NC/8) + q)*64 + l)*2 + h,   r = 16w + l/4,  c = l%4 + 8q + 4h.
-// Rows r >= nr are padding: their matrix rows and columns are zero and their
-// vector entries stay finite (loads are clamped to row nr-1), so they never
-// reach a real row and are written without selects or branches.
+// Rows r >= nr are padding: their matrix columns are zero, their lanes read
+// the operator entries and inputs of row nr-1 (cache lines the valid lanes
+// fetch anyway), so their values stay finite, never reach a real row through
+// the zero columns, and are written to a sink without selects or branches.
 // A matrix-vector product is one partial dot product per lane (NC/4 terms)
 // and a quad reduction (two DPP exchanges, which leave the same sum in all
 // four lanes).  The vector is broadcast through LDS, stored as [c%4][c/4] so
