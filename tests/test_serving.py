@@ -103,3 +103,46 @@ def test_jobs_same_alone_and_in_the_mix():
             assert all(np.diff(ras) > 0), ras
     first = [j for j in jobs if j.kind == "newton"][0]
     assert first.info["converged"], first.info   # from its own steady state
+
+
+@pytest.mark.gpu
+def test_budget_admits_one_class_at_a_time():
+    """With an HBM budget that holds one size class, time jobs of two classes
+    still all finish: a class is released when it is idle and the other needs
+    the memory, at most one class is resident at any time, and every result
+    equals the same job run alone."""
+    import torch
+
+    from spectraldoublediffusiveconvection_amd import device as dev, operators as ops
+    sizes = [(12, 16), (20, 64)]
+    cap = 4
+    budget = max(serving.class_bytes(k, n, cap, "fp64") for n, k in sizes) + 1024
+    rng = np.random.default_rng(7)
+    jobs = []
+    for i in range(6):
+        nr_, k_ = sizes[(i // 2) % 2]
+        x = rng.standard_normal(3 * k_ * (nr_ - 1))
+        jobs.append(serving.Job("time", k_, nr_, float(rng.uniform(2500, 3000)), 500.0,
+                                1e-3 * x / np.linalg.norm(x), nsteps=int(rng.integers(10, 30)),
+                                arrival=float(i)))
+    eng = serving.Engine(precision="fp64", capacity=cap, chunk=5, newton_workers=1,
+                         hbm_budget=budget)
+    resident = []
+    orig = eng._class_for
+
+    def tracked(job):
+        c = orig(job)
+        resident.append(len(eng.classes))
+        return c
+
+    eng._class_for = tracked
+    done = eng.run(jobs, realtime=False)
+    eng.shutdown()
+    assert len(done) == len(jobs)
+    assert resident and max(resident) == 1
+    for j in jobs:
+        plan = dev.Plan(ops.build(j.N_fm, j.N_r, j.d, j.dt, j.Pr, j.Tau), precision="fp64",
+                        max_batch=cap)
+        alone = plan.step(torch.as_tensor(j.X0, device="cuda"), j.Ra, j.Ra_s,
+                          nsteps=j.nsteps)[0].cpu().numpy()
+        assert np.linalg.norm(j.result - alone) / np.linalg.norm(alone) < 1e-12, j.id
