@@ -2,7 +2,10 @@
 
 For one size class and batch size it times HIP-graph-replayed steps under each
 candidate launch shape (rows per chain segment of the solves, instances per
-solve workgroup) and keeps the fastest.  Results are stored in a JSON table
+solve workgroup, sub-batches on forked streams) and keeps the fastest.  The
+graphs hold GRAPH_STEPS steps, as the benchmark's and the sweep's do: the
+sub-batch streams start together at every graph launch, so the length of the
+graph changes how much they overlap.  Results are stored in a JSON table
 keyed by size class, precision and batch bucket; ``TuningTable.lookup``
 serves them to batching.Sweep and the serving engine.
 
@@ -24,6 +27,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(PKG, "tuning_mi355x.json")
 BUCKETS = (1, 4, 16, 64, 256)
+GRAPH_STEPS = 40   # steps per timed graph
 
 
 def bucket(B):
@@ -91,7 +95,7 @@ def _ops_for(N_fm, N_r, d, dt):
     return _OPS[key]
 
 
-def time_shape(N_fm, N_r, B, precision, tuning, steps=10, reps=3, d=0.35, dt=5e-3):
+def time_shape(N_fm, N_r, B, precision, tuning, steps=GRAPH_STEPS, reps=3, d=0.35, dt=5e-3):
     """Best µs per batched step under one launch shape (graph replay)."""
     import torch
 
@@ -114,7 +118,7 @@ def time_shape(N_fm, N_r, B, precision, tuning, steps=10, reps=3, d=0.35, dt=5e-
     return best
 
 
-def tune_class(N_fm, N_r, B, precision="mixed", steps=10):
+def tune_class(N_fm, N_r, B, precision="mixed", steps=GRAPH_STEPS):
     """Best launch shape for one class and batch size: (tuning, {shape: µs})."""
     timings = {}
     best, best_t = None, float("inf")
@@ -134,12 +138,13 @@ def main():
     ap.add_argument("--N_r", type=int, nargs="*", default=[20, 30, 40])
     ap.add_argument("--N_fm", type=int, nargs="*", default=[64, 128, 256, 512])
     ap.add_argument("--batches", type=int, nargs="*", default=list(BUCKETS))
+    ap.add_argument("--steps", type=int, default=GRAPH_STEPS, help="steps per timed graph")
     args = ap.parse_args()
     table = TuningTable(args.out)
     for N_r in args.N_r:
         for N_fm in args.N_fm:
             for B in args.batches:
-                t, tm = tune_class(N_fm, N_r, B, args.precision)
+                t, tm = tune_class(N_fm, N_r, B, args.precision, steps=args.steps)
                 table.put(N_fm, N_r, args.precision, B, t, tm)
                 print("N_r=%d N_fm=%d B=%d -> %s  %s" % (N_r, N_fm, B, t, tm), flush=True)
                 table.save()
