@@ -154,7 +154,7 @@ def cpu_baseline(seconds):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+        print("bench.py: measures one GPU only (instances are independent; no multi-GPU path); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -28,7 +28,7 @@ enum sddc_status {
   SDDC_OK = 0,
   SDDC_ERR_INVALID = 1,      /* bad argument (maps to ValueError)          */
   SDDC_ERR_HIP = 2,          /* HIP runtime error                          */
-  SDDC_ERR_UNSUPPORTED = 3,  /* size or mode this build does not support   */
+  SDDC_ERR_UNSUPPORTED = 3,  /* size or mode the library does not support  */
   SDDC_ERR_ODD_NFM = 4       /* odd N_fm (Matrix_Operators.py:758-759)     */
 };
 
