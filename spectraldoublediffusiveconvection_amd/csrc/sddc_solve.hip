@@ -46,7 +46,6 @@ namespace sddc {
 constexpr int kDepth = 2;    // chain rows of operator data in flight per lane
 constexpr int kDepthE = 2;   // segment exit maps in flight per lane (join)
 constexpr int kDepthM = 4;   // rows in flight in the MFMA kernel (few waves: deep prefetch)
-constexpr int kDepthMixed = 2;   // rows in flight in the mixed-width kernel
 
 struct ChainArgs {
   const double* G;          // right-hand sides, field f of instance b at b*gstride+goff[f]
@@ -452,7 +451,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs 
 // and two ninths (6) of 3's bytes per instance.  Four A4 or eight NAB2
 // instances per workgroup were slower (more per-row work on three
 // wavefronts), as was three NAB2 instances.
-template <int NRS, int NBA, int NBN, bool X2>
+template <int NRS, int NBA, int NBN, bool X2, int KD>
 __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(ChainArgs a) {
   using S = CShape<NRS>;
   constexpr int NBX = NBA > NBN ? NBA : NBN;
@@ -485,9 +484,9 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
     }
     auto* xa = reinterpret_cast<double (*)[NBA][NRS + 64]>(xs);
     if constexpr (X2)
-      local_a4x<NRS, NBA, kDepthMixed>(a, start, k0, k1, in, xa, exit_seg);
+      local_a4x<NRS, NBA, KD>(a, start, k0, k1, in, xa, exit_seg);
     else
-      local_a4<NRS, NBA, kDepthMixed>(a, start, k0, k1, in, xa, exit_seg);
+      local_a4<NRS, NBA, KD>(a, start, k0, k1, in, xa, exit_seg);
     return;
   }
   const Inst<NBN> in = instances<NBN>(a, field, chain);
@@ -502,7 +501,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
       }
     return;
   }
-  local_nab2<NRS, NBN, kDepthMixed>(a, field, start, k0, k1, in,
+  local_nab2<NRS, NBN, KD>(a, field, start, k0, k1, in,
                                     reinterpret_cast<double (*)[NBN][NRS + 64]>(xs), exit_seg);
 }
 
@@ -948,12 +947,12 @@ static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
   return SDDC_OK;
 }
 
-template <int NRS, int NBA, int NBN, bool X2>
+template <int NRS, int NBA, int NBN, bool X2, int KD>
 static int launch_chain_mixed_t(const ChainArgs& a, int B, hipStream_t s) {
   using S = CShape<NRS>;
   constexpr int NBMIN = NBA < NBN ? NBA : NBN;
   const dim3 grid(6, a.nseg, (B + NBMIN - 1) / NBMIN), block(64 * S::WR);
-  hipLaunchKernelGGL((k_chain_local_mixed<NRS, NBA, NBN, X2>), grid, block, 0, s, a);
+  hipLaunchKernelGGL((k_chain_local_mixed<NRS, NBA, NBN, X2, KD>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local_mixed")) return rc;
   if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup
     const dim3 gj(6, a.nseg, (B + 1) / 2);
@@ -968,9 +967,9 @@ static int dispatch_nb(const ChainArgs& a, int NB, int B, hipStream_t s) {
   switch (NB) {
     case 1: return launch_chain_t<NRS, 1>(a, B, s);
     case 2: return launch_chain_t<NRS, 2>(a, B, s);
-    case 3: return launch_chain_mixed_t<NRS, 2, 4, false>(a, B, s);
-    case 5: return launch_chain_mixed_t<NRS, 2, 4, true>(a, B, s);
-    case 6: return launch_chain_mixed_t<NRS, 3, 4, true>(a, B, s);
+    case 3: return launch_chain_mixed_t<NRS, 2, 4, false, 2>(a, B, s);
+    case 5: return launch_chain_mixed_t<NRS, 2, 4, true, 3>(a, B, s);
+    case 6: return launch_chain_mixed_t<NRS, 3, 4, true, 2>(a, B, s);
     case 4: return launch_chain_t<NRS, 4>(a, B, s);
     default: return fail(SDDC_ERR_INVALID, "unsupported instances per workgroup");
   }
