@@ -236,6 +236,30 @@ def test_solve_widths_padded_instances(width):
     p.close()
 
 
+@pytest.mark.parametrize("K,Nr,segment", [(512, 40, 0), (64, 20, 8)])
+def test_solve_width_groups_same_bits(K, Nr, segment):
+    """Solve widths 1-4 give the same bits, as do 5 and 6: within a group the
+    per-row arithmetic and summation order are the same, and the widths differ
+    only in how many instances share a workgroup and how many rows of operator
+    data are in flight (three for width 5, two for the others)."""
+    op = ops.build(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0)
+    nr = Nr - 1
+    rng = np.random.default_rng(K)
+    B = 7
+    g = rng.standard_normal((B, K * nr))
+    X = 1e-3 * rng.standard_normal((B, 3 * K * nr))
+    Ra = np.linspace(2500.0, 3000.0, B)
+    outs = {}
+    for width in (1, 2, 3, 4, 5, 6):
+        p = dev.Plan(op, max_batch=B, tuning={"chain_segment": segment, "solve_instances": width})
+        outs[width] = [H(p.bsub(T(g), f)) for f in range(3)] + [H(p.step(T(X), Ra, 500.0, nsteps=2))]
+        p.close()
+    for group in ((1, 2, 3, 4), (5, 6)):
+        for w in group[1:]:
+            for i, (a, b) in enumerate(zip(outs[group[0]], outs[w])):
+                assert np.array_equal(a, b), (group[0], w, i)
+
+
 def test_single_step_and_linearised(c1):
     o, p = c1
     for name in ("X_big", "X_small"):
