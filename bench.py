@@ -119,8 +119,10 @@ def serving_run(rate, seconds, tuning_table):
                                   ra=(w["Ra_lo"], w["Ra_hi"]), arclength=True)
     eng = serving.Engine(precision="mixed", capacity=64, chunk=25, newton_workers=3,
                          tuning_table=tuning_table)
-    eng.run(jobs, realtime=True, max_seconds=seconds * 10 + 30)
-    eng.shutdown()
+    try:
+        eng.run(jobs, realtime=True, max_seconds=seconds * 10 + 30)
+    finally:   # the worker threads end before the interpreter, also on an error
+        eng.shutdown()
     st = eng.stats()
     st.update({"rate_jobs_per_s": rate, "arrival_window_s": seconds, "submitted": len(jobs),
                "mix": "60% time (500 steps), 30% newton, "
