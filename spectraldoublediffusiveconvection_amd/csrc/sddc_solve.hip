@@ -443,14 +443,18 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs 
 // instances per workgroup (X2: the two-exchange form, one streamed block per
 // row), the NAB2 chains NBN.  The launch has ceil(B / min(NBA, NBN)) groups;
 // workgroups past the end of the batch for their width return at once.
-//   solve_instances 3: NBA 2, NBN 4 (one exchange per A4 row)
-//   solve_instances 5: NBA 2, NBN 4, X2
-//   solve_instances 6: NBA 3, NBN 4, X2
-// Measured at N_fm = 512, N_r = 40, B = 256: the full-batch solve takes
-// 374 µs (3), 362 µs (5) and 303 µs (6); the A4 chains stream a third (5)
-// and two ninths (6) of 3's bytes per instance.  Four A4 or eight NAB2
+//   solve_instances 3: NBA 2, NBN 4 (one exchange per A4 row), KD 2
+//   solve_instances 5: NBA 2, NBN 4, X2, KD 3
+//   solve_instances 6: NBA 3, NBN 4, X2, KD 2
+// KD is the number of rows of operator data in flight; it changes when data
+// arrives, not the arithmetic.  Measured at N_fm = 512, N_r = 40, B = 256
+// with KD 2: the full-batch solve takes 374 µs (3), 362 µs (5) and 303 µs
+// (6); the A4 chains stream a third (5) and two ninths (6) of 3's bytes per
+// instance.  Inside the four-stream step graph, width 5 with KD 3 is the
+// fastest (567 µs per step against 613 µs for width 6 with KD 2; KD 3 made
+// width 6 slower and KD 4 made width 5 slower).  Four A4 or eight NAB2
 // instances per workgroup were slower (more per-row work on three
-// wavefronts), as was three NAB2 instances.
+// wavefronts), as were three NAB2 instances or one A4 instance with KD 3.
 template <int NRS, int NBA, int NBN, bool X2, int KD>
 __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(ChainArgs a) {
   using S = CShape<NRS>;
