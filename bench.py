@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--graph-steps", type=int, default=40)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-agg-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -130,27 +131,124 @@ def serving_run(rate, seconds, tuning_table):
     return st
 
 
-def cpu_baseline(seconds):
-    """The oracle's float64 NumPy step (reference algorithm restated) on one
-    host core, over a bounded sample of the same workload."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_steps(seconds, seed=1234):
+    """(steps, seconds) of the oracle's float64 NumPy step on one thread: the
+    reference timer's definition, step only, first step untimed (Main.py:285-290,
+    325-327)."""
     from threadpoolctl import threadpool_limits
 
     from oracle import sddc_oracle as O
     w = WORKLOAD
     o = O.build_operators(w["N_fm"], w["N_r"], w["d"], w["dt"], w["Pr"], w["Tau"])
-    X = seeded_states(1, 3 * w["N_fm"] * (w["N_r"] - 1))[0]
+    X = seeded_states(1, 3 * w["N_fm"] * (w["N_r"] - 1), seed=seed)[0]
     with threadpool_limits(1):
-        X = O.step(X, o, w["Ra_hi"], w["Ra_s"])   # first step untimed (reference timer, Main.py:325)
+        X = O.step(X, o, w["Ra_hi"], w["Ra_s"])
         n, t0 = 0, time.perf_counter()
         while True:
             X = O.step(X, o, w["Ra_hi"], w["Ra_s"])
             n += 1
             el = time.perf_counter() - t0
             if el >= seconds and n >= 5:
-                break
-    return {"value": n / el, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": "%d float64 steps of one N_r=40, N_fm=512 instance (NumPy restatement "
-                      "of the reference step, one thread), %.1f s" % (n, el)}
+                return n, el
+
+
+def _cpu_worker(args):
+    seconds, seed = args
+    return _cpu_steps(seconds, seed)
+
+
+def cpu_baseline(seconds, agg_seconds):
+    """The oracle's float64 NumPy step (reference algorithm restated) on the
+    host: one instance on one thread, then P single-thread processes (one
+    instance each) at once, P = the CPUs this process may run on, at most 16
+    (the host share of one GPU).  Runs before the GPU is touched, so the
+    worker processes are forked from a process without GPU state."""
+    import multiprocessing as mp
+    n1, el1 = _cpu_steps(seconds)
+    affinity = len(os.sched_getaffinity(0))
+    P = max(1, min(affinity, 16))
+    with mp.get_context("fork").Pool(P) as pool:
+        res = pool.map(_cpu_worker, [(agg_seconds, 1234 + i) for i in range(P)])
+    agg = sum(n / el for n, el in res)
+    return {"value": agg, "unit": "steps/s", "cores": P, "kind": "port",
+            "single_core_steps_per_s": n1 / el1,
+            "cpu_model": _cpu_model(), "affinity_cpus": affinity,
+            "sample": "aggregate of %d single-thread processes, one N_r=40, N_fm=512 instance "
+                      "each, %.0f s each (%d steps in all); single core: %d float64 steps in "
+                      "%.1f s (NumPy restatement of the reference step)"
+                      % (P, agg_seconds, sum(n for n, _ in res), n1, el1)}
+
+
+THIN_GAP = dict(d=0.353, Ra=2360.0, Ra_s=0.0, Pr=1.0, Tau=1.0)
+
+
+def convecting_state(dev, torch, K, Nr, seed=1234, steps=30000):
+    """Steady convecting state of the reference's thin-gap validation case
+    (Run_Tests.py:170-196: d=0.353, Ra_T=2360, Ra_S=0, Pr=1, Tau=1) at (K, Nr):
+    a seeded state of norm 1e-3, equatorially symmetric as the reference runs
+    the case, stepped with dt=0.075 until steady (the fixed point of the step
+    does not depend on dt)."""
+    c = THIN_GAP
+    sp = dev.Plan.build(K, Nr, c["d"], 0.075, c["Pr"], c["Tau"], symmetric=True)
+    x0 = np.random.default_rng(seed).random(3 * K * (Nr - 1))
+    X = torch.as_tensor(1e-3 * x0 / np.linalg.norm(x0), device="cuda")[None]
+    X = sp.step(X, c["Ra"], c["Ra_s"], nsteps=steps)
+    Y = sp.step(X, c["Ra"], c["Ra_s"], nsteps=100)
+    drift = float((Y - X).norm() / X.norm())
+    sp.close()
+    return Y, drift
+
+
+def mixed_vs_fp64(dev, torch):
+    """Mixed precision against float64 at N_r=40, N_fm=512 in convecting flow:
+    kinetic energy and Nu - 1 (T and S at R_1) every 100 steps over 10^4 steps
+    of dt=5e-3 (symmetric=False), from the thin-gap steady convecting state."""
+    w, c = WORKLOAD, THIN_GAP
+    K, Nr, dt = w["N_fm"], w["N_r"], w["dt"]
+    X, drift = convecting_state(dev, torch, K, Nr)
+    p64 = dev.Plan.build(K, Nr, c["d"], dt, c["Pr"], c["Tau"])
+    p32 = dev.Plan.build(K, Nr, c["d"], dt, c["Pr"], c["Tau"], precision="mixed")
+    ratio = dt * float(p64.nlin_fx(X).norm() / X.norm())
+    d0 = p64.diagnostics(X)[0].cpu().numpy()
+    x64, x32 = X.clone(), X.clone()
+    worst, worst_state = np.zeros(3), 0.0
+    for _ in range(100):
+        x64 = p64.step(x64, c["Ra"], c["Ra_s"], nsteps=100)
+        x32 = p32.step(x32, c["Ra"], c["Ra_s"], nsteps=100)
+        d64 = p64.diagnostics(x64)[0, 1:4].cpu().numpy()
+        d32 = p64.diagnostics(x32)[0, 1:4].cpu().numpy()
+        worst = np.maximum(worst, np.abs(d32 / d64 - 1.0))
+        worst_state = max(worst_state, float((x32 - x64).norm() / x64.norm()))
+    return {"regime": "thin-gap steady convection (Run_Tests.py:170-196): d=0.353 Ra_T=2360 "
+                      "Ra_S=0 Pr=1 Tau=1, dt=5e-3, symmetric=False",
+            "dt_F_over_X": ratio, "kinetic_energy": float(d0[1]), "nu_t_minus_1": float(d0[2]),
+            "spinup_drift_per_100_steps": drift,
+            "steps": 10000, "every": 100, "ke_rel_max": float(worst[0]),
+            "nu_t_minus_1_rel_max": float(worst[1]), "nu_s_minus_1_rel_max": float(worst[2]),
+            "state_rel_max": worst_state}
+
+
+def time_graph(graph, steps, gsteps, warmup, torch):
+    for _ in range(max(1, warmup // gsteps)):
+        graph.launch()
+    torch.cuda.synchronize()
+    nl = max(1, steps // gsteps)
+    t0 = time.perf_counter()
+    for _ in range(nl):
+        graph.launch()
+    torch.cuda.synchronize()
+    return nl * gsteps, time.perf_counter() - t0
 
 
 def main():
@@ -159,6 +257,10 @@ def main():
         print("bench.py: measures one GPU only (instances are independent; no multi-GPU path); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    # the CPU baseline first: its worker processes fork from a process that
+    # has not touched the GPU yet
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, args.cpu_agg_seconds)
+
     import torch
 
     from spectraldoublediffusiveconvection_amd import autotune
@@ -173,28 +275,37 @@ def main():
     Ra = np.linspace(w["Ra_lo"], w["Ra_hi"], B)
     table = autotune.TuningTable()
     tuning = table.lookup((w["N_fm"], w["N_r"]), B, "mixed")
+    gsteps = max(1, min(args.graph_steps, args.steps))
 
     # ---- batched mixed-precision step, graph-replayed
     plan = dev.Plan(op, precision="mixed", max_batch=B, tuning=tuning)
     X = torch.as_tensor(seeded_states(B, n3), device="cuda")
-    gsteps = max(1, min(args.graph_steps, args.steps))
     graph = plan.graph(X, Ra, w["Ra_s"], nsteps=gsteps)
-    for _ in range(max(1, args.warmup // gsteps)):
-        graph.launch()
-    torch.cuda.synchronize()
-    nl = max(1, args.steps // gsteps)
-    steps = nl * gsteps
-    t0 = time.perf_counter()
-    for _ in range(nl):
-        graph.launch()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    steps, el = time_graph(graph, args.steps, gsteps, args.warmup, torch)
     finite = bool(torch.isfinite(X).all().item())
     value = B * steps / el
+    graph.close()
 
-    # ---- per-kernel timing of the dominant kernel (HIP events on its stream)
-    roof = profiling.roofline(plan, X, Ra, w["Ra_s"], reps=20)
-    roof["traffic"] = measured_traffic(roof["kernel"], B)
+    # ---- the same batch in float64 (batching alone, without the precision split)
+    plan_f64 = dev.Plan(op, precision="fp64", max_batch=B, tuning=tuning)
+    Xd = torch.as_tensor(seeded_states(B, n3), device="cuda")
+    gd = plan_f64.graph(Xd, Ra, w["Ra_s"], nsteps=gsteps)
+    steps_d, el_d = time_graph(gd, args.steps, gsteps, args.warmup, torch)
+    batched_fp64 = {"instance_steps_per_s": B * steps_d / el_d, "ms_per_step": 1e3 * el_d / steps_d,
+                    "states_finite": bool(torch.isfinite(Xd).all().item())}
+    gd.close()
+    plan_f64.close()
+
+    # ---- per-kernel timing of the dominant kernel (HIP events on its stream):
+    # at the sub-batch shape the graph runs, and on the whole batch
+    parts = max(1, (plan.tuning() or {}).get("sub_batches", 1))
+    sub = B // parts if B % parts == 0 else B
+    roof = profiling.roofline(plan, X[:sub], Ra[:sub], w["Ra_s"], reps=20)
+    roof["traffic"] = measured_traffic(roof["kernel"], sub)
+    roof["batch"] = sub
+    roof_full = profiling.roofline(plan, X, Ra, w["Ra_s"], reps=20)
+    roof_full["traffic"] = measured_traffic(roof_full["kernel"], B)
+    roof_full["batch"] = B
 
     # ---- plain float64 single-instance step (eager launches)
     p64 = dev.Plan(op)
@@ -209,21 +320,7 @@ def main():
     el64 = time.perf_counter() - t1
     plain = n64 / el64
 
-    # ---- mixed method against float64: kinetic energy and Nusselt numbers
-    # every 100 steps over 10^4 steps of one instance
-    p32 = dev.Plan(op, precision="mixed")
-    x64 = torch.as_tensor(seeded_states(1, n3, seed=5), device="cuda")
-    x32 = x64.clone()
-    worst = np.zeros(3)
-    for _ in range(100):
-        x64 = p64.step(x64, w["Ra_hi"], w["Ra_s"], nsteps=100)
-        x32 = p32.step(x32, w["Ra_hi"], w["Ra_s"], nsteps=100)
-        d64 = p64.diagnostics(x64)[0, 1:4].cpu().numpy()
-        d32 = p64.diagnostics(x32)[0, 1:4].cpu().numpy()
-        worst = np.maximum(worst, np.abs(d32 / d64 - 1.0))
-    mixed_err = {"steps": 10000, "every": 100, "ke_rel_max": float(worst[0]),
-                 "nu_t_minus_1_rel_max": float(worst[1]), "nu_s_minus_1_rel_max": float(worst[2])}
-
+    batched_fp64["speedup_vs_plain_fp64"] = batched_fp64["instance_steps_per_s"] / plain
     line = {
         "metric": "aggregate CN/AB2 steps/s at Nr=40,Ntheta=512 vs plain FP64 GPU; p50/p99 job latency",
         "value": value,
@@ -243,8 +340,11 @@ def main():
                    "launch_shape": plan.tuning()},
         "plain_fp64_gpu": {"steps_per_s": plain, "ms_per_step": 1e3 * el64 / n64},
         "speedup_vs_plain_fp64": value / plain,
-        "mixed_vs_fp64": mixed_err,
+        "batched_fp64_graph": batched_fp64,
+        "mixed_over_batched_fp64": value / batched_fp64["instance_steps_per_s"],
+        "mixed_vs_fp64": mixed_vs_fp64(dev, torch),
         "roofline": roof,
+        "roofline_full_batch": roof_full,
     }
     if not args.no_sweep:
         line["sweep_mixed_sizes"] = sweep_c4(table)
@@ -256,8 +356,8 @@ def main():
             line["serving_rates"].append({k: st[k] for k in (
                 "rate_jobs_per_s", "submitted", "jobs", "jobs_per_s", "latency_p50_ms",
                 "latency_p99_ms", "instance_steps_per_s")})
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line))
 
 

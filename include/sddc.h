@@ -6,8 +6,16 @@
  * void*; NULL = the default stream) and are stream-ordered.  They return an
  * int status (SDDC_OK = 0); sddc_last_error() gives the message of the last
  * failure on the calling thread.  No entry point allocates device memory
- * except sddc_plan_create / sddc_plan_reserve and the first call of a
- * transform at a new length (its twiddle tables are cached).
+ * except sddc_plan_create / sddc_plan_reserve, sddc_krylov_create and the
+ * first call of a transform at a new length (its twiddle tables are cached).
+ *
+ * Threads: one device-state lock (process-wide) is held for the whole of a
+ * graph capture (sddc_graph_create*) and by every call that allocates,
+ * frees, copies synchronously or synchronises the device (plan and Krylov
+ * create/reserve/destroy, graph destroy, new transform lengths), so such a
+ * call on another thread waits for a capture to finish instead of
+ * invalidating it.  Compute calls take no lock; calls on one plan (or one
+ * Krylov object) must not overlap in time on different threads.
  *
  * Batches: a plan describes one size class (N_fm, N_r, d, dt, Pr, Tau); B
  * instances of that class are stored back to back, each in the reference's
@@ -101,6 +109,10 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
                            int max_batch, const sddc_tuning* tuning, sddc_plan** out);
 /* The launch shapes a plan uses (defaults resolved). */
 int sddc_plan_tuning(const sddc_plan* plan, sddc_tuning* out);
+/* Grow the plan's workspace to max_batch instances (no-op if it holds that
+ * many).  Growing frees the old workspace, so it is refused with
+ * SDDC_ERR_INVALID while a graph created on the plan is alive; so is
+ * sddc_plan_destroy.  Destroy the graphs first. */
 int sddc_plan_reserve(sddc_plan* plan, int max_batch);
 int sddc_plan_destroy(sddc_plan* plan);
 int sddc_plan_info(const sddc_plan* plan, long long* workspace_bytes, int* lds_nonlinear);

@@ -14,6 +14,7 @@
 
 
 struct sddc_graph {
+  std::vector<sddc_plan*> plans;   // plans whose workspace the graph holds (one count each)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t own = nullptr;
@@ -24,6 +25,11 @@ struct sddc_graph {
 namespace sddc {
 
 static thread_local std::string g_last_error;
+
+std::recursive_mutex& device_state_mutex() {
+  static std::recursive_mutex mu;
+  return mu;
+}
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
@@ -223,6 +229,7 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
                            const sddc_tuning* tuning, sddc_plan** out) {
   if (!ops || !out) return fail(SDDC_ERR_INVALID, "null argument");
   *out = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
   if (ops->N_fm % 2 != 0) {
     char buf[96];
     snprintf(buf, sizeof(buf), "The number of Fourier modes is not even %d", ops->N_fm);
@@ -309,12 +316,24 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
 
 int sddc_plan_reserve(sddc_plan* plan, int max_batch) {
   if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
-  SDDC_HIP(hipDeviceSynchronize());
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
+  if (max_batch <= plan->p.max_batch) return SDDC_OK;
+  if (plan->live_graphs > 0) {
+    char buf[160];
+    snprintf(buf, sizeof(buf),
+             "the plan's workspace is held by %d HIP graph(s); destroy them before reserving "
+             "%d instances (reserved: %d)", plan->live_graphs, max_batch, plan->p.max_batch);
+    return fail(SDDC_ERR_INVALID, buf);
+  }
+  SDDC_HIP(hipDeviceSynchronize());   // the old workspace may still be in use by queued kernels
   return alloc_ws(plan->p, max_batch);
 }
 
 int sddc_plan_destroy(sddc_plan* plan) {
   if (!plan) return SDDC_OK;
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
+  if (plan->live_graphs > 0)
+    return fail(SDDC_ERR_INVALID, "the plan's workspace is held by live HIP graphs");
   Plan& p = plan->p;
   free_ws(p);
   void* bufs[] = {p.Dfull, p.R, p.mfma64, p.mfma32, p.vecs};
@@ -348,6 +367,7 @@ int sddc_transform(int kind, const double* in, double* out, int rows, int len_in
   SDDC_HIP(hipGetDevice(&dev));
   TableSet* t = nullptr;
   {
+    std::lock_guard<std::recursive_mutex> dlk(device_state_mutex());
     std::lock_guard<std::mutex> lk(mu);
     auto key = std::make_pair(dev, L);
     auto it = cache.find(key);
@@ -613,6 +633,9 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
     if (!plans[c]) return fail(SDDC_ERR_INVALID, "null plan");
     if (int rc = check_batch(plans[c]->p, B[c])) return rc;
   }
+  // no allocation, free or device synchronisation of this library may run
+  // on another thread between the capture's begin and end
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
   sddc_graph* g = new sddc_graph();
   auto bail = [&](int rc) {
     sddc_graph_destroy(g);
@@ -668,6 +691,14 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
   rc = rc2 ? rc2 : hip_check(e, "end capture");
   if (!rc) rc = hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "instantiate");
   if (rc) return bail(rc);
+  for (int c = 0; c < nplans; ++c) {
+    bool seen = false;
+    for (sddc_plan* q : g->plans) seen = seen || q == plans[c];
+    if (!seen) {
+      g->plans.push_back(plans[c]);
+      ++plans[c]->live_graphs;
+    }
+  }
   *out = g;
   return SDDC_OK;
 }
@@ -680,6 +711,8 @@ int sddc_graph_launch(sddc_graph* g, void* stream) {
 
 int sddc_graph_destroy(sddc_graph* g) {
   if (!g) return SDDC_OK;
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
+  for (sddc_plan* q : g->plans) --q->live_graphs;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   if (g->own) (void)hipStreamDestroy(g->own);

@@ -13,6 +13,7 @@
 // nonlinear-pipeline precision (float64 plain, float32 mixed).
 #pragma once
 #include <atomic>
+#include <mutex>
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -150,6 +151,15 @@ int hip_check(hipError_t e, const char* what);
     if (_rc != SDDC_OK) return _rc;                     \
   } while (0)
 
+// The device-state lock.  Held for the whole of a HIP graph capture and by
+// every library call that allocates, frees, copies synchronously to or from
+// the device or synchronises the device (plan create/reserve/destroy,
+// Krylov create/destroy, graph destroy, transform tables, kernel attribute
+// changes).  Such calls from another thread during a capture can invalidate
+// it (hipErrorStreamCaptureInvalidated at hipStreamEndCapture) even in
+// thread-local capture mode; under the lock they wait for the capture to end.
+std::recursive_mutex& device_state_mutex();
+
 int build_tables(TableSet& t, int L);
 void free_tables(TableSet& t);
 FftShape make_shape(int L);
@@ -183,5 +193,6 @@ int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& curren
 // The opaque handle of include/sddc.h.
 struct sddc_plan {
   sddc::Plan p;
+  int live_graphs = 0;   // graphs holding this plan's workspace pointers (under the lock)
 };
 

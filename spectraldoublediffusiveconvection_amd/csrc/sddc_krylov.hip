@@ -13,7 +13,9 @@
 // a fixed tree, and the per-workgroup partials in a fixed order, so a solve
 // gives the same bits on every run.  The small Hessenberg least-squares
 // problem (Givens rotations, back substitution) is solved on the host from
-// the j+1 projections copied back per Arnoldi step.
+// the j+1 projections copied back per Arnoldi step into pinned host memory
+// (no allocation, free or device-wide synchronisation after creation).
+#include <algorithm>
 #include <cmath>
 #include <functional>
 #include <vector>
@@ -33,7 +35,10 @@ struct sddc_krylov {
   double* scal = nullptr;      // device scalars: 1, 0, Ra of the linearisation, a dot product
   double* partial = nullptr;   // (m+2) x kBlocks partial sums
   double* h = nullptr;         // 2(m+2)+1: first and second Gram-Schmidt projections, a norm
-  std::vector<double> hh;      // host copy
+  // pinned host mirror (hipHostMalloc, 2(m+2)+4 doubles): projections, then
+  // the scalar slot of norm2/get_scalar; every copy is an async DMA on the
+  // caller's stream followed by a wait on that stream only
+  double* hpin = nullptr;
 };
 
 namespace sddc {
@@ -123,13 +128,13 @@ static int dots(sddc_krylov& k, const double* V, const double* x, int len, int n
 
 static double* proj2(sddc_krylov& k) { return k.h + (k.m + 2); }
 static double* norm_slot(sddc_krylov& k) { return k.h + 2 * (k.m + 2); }
+static double* host_scalar(sddc_krylov& k) { return k.hpin + 2 * (k.m + 2); }
 
 static int norm2(sddc_krylov& k, const double* x, int len, double* host, hipStream_t s) {
   if (int rc = dots(k, nullptr, x, len, 1, 0, norm_slot(k), s)) return rc;
-  double v = 0.0;
-  SDDC_HIP(hipMemcpyAsync(&v, norm_slot(k), sizeof(double), hipMemcpyDeviceToHost, s));
+  SDDC_HIP(hipMemcpyAsync(host_scalar(k), norm_slot(k), sizeof(double), hipMemcpyDeviceToHost, s));
   SDDC_HIP(hipStreamSynchronize(s));
-  *host = std::sqrt(v);
+  *host = std::sqrt(*host_scalar(k));
   return SDDC_OK;
 }
 
@@ -174,9 +179,10 @@ static int set_scalar(double* x, double value, hipStream_t s) {
   return hip_check(hipGetLastError(), "k_set");
 }
 
-static int get_scalar(const double* x, double* host, hipStream_t s) {
-  SDDC_HIP(hipMemcpyAsync(host, x, sizeof(double), hipMemcpyDeviceToHost, s));
+static int get_scalar(sddc_krylov& k, const double* x, double* host, hipStream_t s) {
+  SDDC_HIP(hipMemcpyAsync(host_scalar(k), x, sizeof(double), hipMemcpyDeviceToHost, s));
   SDDC_HIP(hipStreamSynchronize(s));
+  *host = *host_scalar(k);
   return SDDC_OK;
 }
 
@@ -225,13 +231,12 @@ static int gmres_core(sddc_krylov& k, int len, const ApplyFn& A, const double* b
       hipLaunchKernelGGL(k_combine, dim3(grid_for(len)), dim3(256), 0, s, vn, k.V, proj2(k), len,
                          ld, j + 1, 1.0, -1.0);
       if ((rc = hip_check(hipGetLastError(), "k_combine"))) return rc;
-      SDDC_HIP(hipMemcpyAsync(k.hh.data(), k.h, (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
-      std::vector<double> h2(j + 1);
-      SDDC_HIP(hipMemcpyAsync(h2.data(), proj2(k), (j + 1) * sizeof(double),
+      // both projection sets in one copy (h holds them back to back)
+      SDDC_HIP(hipMemcpyAsync(k.hpin, k.h, 2 * (size_t)(m + 2) * sizeof(double),
                               hipMemcpyDeviceToHost, s));
       double hn;
-      if ((rc = norm2(k, vn, len, &hn, s))) return rc;   // synchronises the copies too
-      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = k.hh[i] + h2[i];
+      if ((rc = norm2(k, vn, len, &hn, s))) return rc;   // synchronises the copy too
+      for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = k.hpin[i] + k.hpin[(m + 2) + i];
       H[(size_t)(j + 1) * m + j] = hn;
       if (hn > 0.0 && (rc = axpby(vn, vn, nullptr, len, 1.0 / hn, 0.0, s))) return rc;
       // Givens rotations of the new column
@@ -261,11 +266,12 @@ static int gmres_core(sddc_krylov& k, int len, const ApplyFn& A, const double* b
       for (int q = i + 1; q < j; ++q) acc -= H[(size_t)i * m + q] * y[q];
       y[i] = acc / H[(size_t)i * m + i];
     }
-    SDDC_HIP(hipMemcpyAsync(k.h, y.data(), j * sizeof(double), hipMemcpyHostToDevice, s));
+    std::copy(y.begin(), y.begin() + j, k.hpin);
+    SDDC_HIP(hipMemcpyAsync(k.h, k.hpin, j * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_combine, dim3(grid_for(len)), dim3(256), 0, s, x, k.V, k.h, len, ld, j,
                        1.0, 1.0);
     if ((rc = hip_check(hipGetLastError(), "k_combine"))) return rc;
-    SDDC_HIP(hipStreamSynchronize(s));   // y is a host buffer
+    SDDC_HIP(hipStreamSynchronize(s));   // the copy reads the pinned buffer
     if (rnorm <= target) {
       converged = 1;
       ++restarts;
@@ -309,11 +315,11 @@ int sddc_krylov_create(sddc_plan* plan, int restart, sddc_krylov** out) {
   if (!plan || !out) return fail(SDDC_ERR_INVALID, "null argument");
   if (restart < 1 || restart > 2000) return fail(SDDC_ERR_INVALID, "restart must be in [1, 2000]");
   *out = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
   sddc_krylov* k = new sddc_krylov();
   k->plan = plan;
   k->n = 3 * plan->p.K * plan->p.nr;
   k->m = restart;
-  k->hh.assign(restart + 2, 0.0);
   const size_t n1 = (size_t)k->n + 1;
   auto A = [&](double** p, size_t count) {
     return hip_check(hipMalloc((void**)p, count * sizeof(double)), "krylov workspace");
@@ -327,9 +333,14 @@ int sddc_krylov_create(sddc_plan* plan, int restart, sddc_krylov** out) {
   if (!rc) rc = A(&k->scal, 4);
   if (!rc) rc = A(&k->partial, (size_t)(restart + 2) * kBlocks);
   if (!rc) rc = A(&k->h, 2 * ((size_t)restart + 2) + 1);
+  if (!rc)
+    rc = hip_check(hipHostMalloc((void**)&k->hpin, (2 * ((size_t)restart + 2) + 4) * sizeof(double),
+                                 hipHostMallocDefault),
+                   "krylov pinned host buffer");
   if (!rc) {
-    const double one_zero[2] = {1.0, 0.0};
-    rc = hip_check(hipMemcpy(k->scal, one_zero, sizeof(one_zero), hipMemcpyHostToDevice),
+    k->hpin[0] = 1.0;
+    k->hpin[1] = 0.0;
+    rc = hip_check(hipMemcpy(k->scal, k->hpin, 2 * sizeof(double), hipMemcpyHostToDevice),
                    "krylov constants");
   }
   if (!rc && plan->p.max_batch < 1) rc = sddc_plan_reserve(plan, 1);
@@ -343,10 +354,12 @@ int sddc_krylov_create(sddc_plan* plan, int restart, sddc_krylov** out) {
 
 int sddc_krylov_destroy(sddc_krylov* k) {
   if (!k) return SDDC_OK;
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
   double* bufs[] = {k->V, k->w, k->r, k->f, k->t, k->c[0], k->c[1], k->c[2], k->c[3], k->c[4],
                     k->scal, k->partial, k->h};
   for (double* b : bufs)
     if (b) (void)hipFree(b);
+  if (k->hpin) (void)hipHostFree(k->hpin);
   delete k;
   return SDDC_OK;
 }
@@ -438,7 +451,7 @@ int sddc_arclength_step(sddc_krylov* k, const double* Ra_s, double* Y, double* Y
   // Predict (Main.py:838-873)
   if ((rc = masked_copy(*k, Y, X0, s))) return rc;
   double mu0, ydn;
-  if ((rc = get_scalar(Y + n, &mu0, s))) return rc;
+  if ((rc = get_scalar(*k, Y + n, &mu0, s))) return rc;
   if ((rc = norm2(*k, Y_dot, n + 1, &ydn, s))) return rc;
   if (ydn == 0.0 || k->plan->p.dt < 10.0) {
     // xi solves DF_X(X_0) xi = -DF_mu(X_0)
@@ -465,7 +478,7 @@ int sddc_arclength_step(sddc_krylov* k, const double* Ra_s, double* Y, double* Y
     if ((rc = axpby(Xdot, dY, nullptr, n, mudot, 0.0, s))) return rc;
   } else {
     if ((rc = axpby(Xdot, Y_dot, nullptr, n, 1.0, 0.0, s))) return rc;
-    if ((rc = get_scalar(Y_dot + n, &mudot, s))) return rc;
+    if ((rc = get_scalar(*k, Y_dot + n, &mudot, s))) return rc;
   }
   double step = *ds;
   auto predict = [&]() {
@@ -493,7 +506,7 @@ int sddc_arclength_step(sddc_krylov* k, const double* Ra_s, double* Y, double* Y
     }
     double mu;
     if ((rc = masked_copy(*k, Y, Xc, s))) return rc;
-    if ((rc = get_scalar(Y + n, &mu, s))) return rc;
+    if ((rc = get_scalar(*k, Y + n, &mu, s))) return rc;
     if ((rc = set_scalar(mu_c, mu, s))) return rc;
     if ((rc = pdfmu(*k, Xc, dfmu, s))) return rc;
     // G = (Step(Xc) - Xc, delta X_dot.(Xc - X_0) + (1 - delta) mu_dot (mu - mu_0) - ds)
@@ -502,11 +515,11 @@ int sddc_arclength_step(sddc_krylov* k, const double* Ra_s, double* Y, double* Y
     if ((rc = axpby(k->r, Xc, X0, n, 1.0, -1.0, s))) return rc;
     if ((rc = dots(*k, Xdot, k->r, n, 1, 0, dot, s))) return rc;
     double xdot_dx, gx2;
-    if ((rc = get_scalar(dot, &xdot_dx, s))) return rc;
+    if ((rc = get_scalar(*k, dot, &xdot_dx, s))) return rc;
     const double gmu = delta * xdot_dx + (1.0 - delta) * mudot * (mu - mu0) - step;
     if ((rc = set_scalar(k->f + n, gmu, s))) return rc;
     if ((rc = dots(*k, nullptr, k->f, n, 1, 0, dot, s))) return rc;
-    if ((rc = get_scalar(dot, &gx2, s))) return rc;
+    if ((rc = get_scalar(*k, dot, &gx2, s))) return rc;
     const double bn = std::sqrt(delta * gx2 + (1.0 - delta) * gmu * gmu);
     go.atol = opts->tol_gmres * bn;   // Main.py:917-918
     if ((rc = axpby(dY, dY, nullptr, n + 1, 0.0, 0.0, s))) return rc;
@@ -524,7 +537,7 @@ int sddc_arclength_step(sddc_krylov* k, const double* Ra_s, double* Y, double* Y
     double ndx, nx, dmu;
     if ((rc = norm2(*k, dY, n, &ndx, s))) return rc;
     if ((rc = norm2(*k, Xc, n, &nx, s))) return rc;
-    if ((rc = get_scalar(dY + n, &dmu, s))) return rc;
+    if ((rc = get_scalar(*k, dY + n, &dmu, s))) return rc;
     err_X = ndx / nx;
     err_mu = std::fabs(dmu) / std::fabs(mu);
     ++it;
@@ -552,8 +565,8 @@ int sddc_arclength_step(sddc_krylov* k, const double* Ra_s, double* Y, double* Y
   info->tangent_ok = 1;
   double t2, tmu;
   if ((rc = dots(*k, nullptr, Y_dot, n, 1, 0, dot, s))) return rc;
-  if ((rc = get_scalar(dot, &t2, s))) return rc;
-  if ((rc = get_scalar(Y_dot + n, &tmu, s))) return rc;
+  if ((rc = get_scalar(*k, dot, &t2, s))) return rc;
+  if ((rc = get_scalar(*k, Y_dot + n, &tmu, s))) return rc;
   const double scale = std::sqrt(delta * t2 + (1.0 - delta) * tmu * tmu);
   if ((rc = axpby(Y_dot, Y_dot, nullptr, n + 1, 1.0 / scale, 0.0, s))) return rc;
   return hip_check(hipStreamSynchronize(s), "arclength step");

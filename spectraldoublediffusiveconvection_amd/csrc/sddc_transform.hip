@@ -958,6 +958,8 @@ static int nl_inplace_nc(const Plan& p) {
 // threads (the attribute is idempotent, the record is atomic).
 int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& current) {
   if (bytes <= 64 * 1024 || bytes <= current.load(std::memory_order_acquire)) return SDDC_OK;
+  std::lock_guard<std::recursive_mutex> lk(device_state_mutex());
+  if (bytes <= current.load(std::memory_order_acquire)) return SDDC_OK;
   const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();   // do not leave the error for the next launch

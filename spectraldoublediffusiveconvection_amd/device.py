@@ -34,6 +34,24 @@ def require_gpu():
         raise RuntimeError("libsddc needs a ROCm GPU (torch.cuda.is_available() is False)")
 
 
+def check_out(out, X):
+    """The output tensor of a call on the batch X: a new one when out is None,
+    else the caller's, which must be a contiguous float64 tensor on X's device
+    with exactly X.numel() values (the kernels write B*3N doubles through its
+    data pointer).  Raises ValueError otherwise."""
+    if out is None:
+        return torch.empty_like(X)
+    if not isinstance(out, torch.Tensor) or out.dtype != torch.float64:
+        raise ValueError("out must be a float64 tensor")
+    if out.device != X.device:
+        raise ValueError("out is on %s, the states on %s" % (out.device, X.device))
+    if not out.is_contiguous():
+        raise ValueError("out must be contiguous")
+    if out.numel() != X.numel():
+        raise ValueError("out has %d values, the batch needs %d" % (out.numel(), X.numel()))
+    return out
+
+
 class Plan:
     """Operators of one size class uploaded once (Build_Matrix_Operators,
     Main.py:179-225) plus workspace for up to ``max_batch`` instances."""
@@ -89,8 +107,9 @@ class Plan:
                 "sub_batches": t.sub_batches}
 
     def close(self):
+        """Free the plan (refused, ValueError, while a graph holds its workspace)."""
         if getattr(self, "_h", None):
-            _lib.lib().sddc_plan_destroy(self._h)
+            _lib.check(_lib.lib().sddc_plan_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -114,6 +133,10 @@ class Plan:
     def _state(self, X):
         if not (isinstance(X, torch.Tensor) and X.is_cuda and X.dtype == torch.float64):
             raise TypeError("states must be float64 CUDA tensors")
+        if X.device != self.device:
+            raise ValueError("states are on %s, the plan on %s" % (X.device, self.device))
+        if X.dim() not in (1, 2):
+            raise ValueError("states must be (3N,) or (B, 3N)")
         X = X.contiguous()
         if X.dim() == 1:
             X = X.view(1, -1)
@@ -121,6 +144,9 @@ class Plan:
             raise ValueError("state length %d != 3*N_fm*nr = %d" % (X.shape[-1], 3 * self.N))
         self.reserve(X.shape[0])
         return X
+
+    def _out(self, out, X):
+        return check_out(out, X)
 
     def _params(self, v, B):
         if isinstance(v, torch.Tensor):
@@ -140,18 +166,20 @@ class Plan:
         mask (Main.py:323).  X: (B, 3N) float64 CUDA tensor."""
         X = self._state(X)
         B = X.shape[0]
-        out = torch.empty_like(X) if out is None else out
+        out = self._out(out, X)
         ra, ras = self._params(Ra, B), self._params(Ra_s, B)
         _lib.check(_lib.lib().sddc_step(self._h, B, _dev_ptr(ra), _dev_ptr(ras), _dev_ptr(X),
                                         _dev_ptr(out), int(nsteps), int(bool(linear)), _stream()))
         return out
 
-    def step_linearized(self, dv, X, Ra, Ra_s):
+    def step_linearized(self, dv, X, Ra, Ra_s, out=None):
         """PDFX without '- dv' (Main.py:498-521)."""
         X = self._state(X)
         dv = self._state(dv)
         B = X.shape[0]
-        out = torch.empty_like(X)
+        if dv.shape != X.shape:
+            raise ValueError("dv and X must have the same shape")
+        out = self._out(out, X)
         ra, ras = self._params(Ra, B), self._params(Ra_s, B)
         _lib.check(_lib.lib().sddc_step_linearized(self._h, B, _dev_ptr(ra), _dev_ptr(ras),
                                                    _dev_ptr(dv), _dev_ptr(X), _dev_ptr(out),
@@ -225,6 +253,10 @@ class StepGraph:
 
     def __init__(self, plan, X, Ra, Ra_s, nsteps):
         self.plan = plan
+        if not (isinstance(X, torch.Tensor) and X.is_contiguous()):
+            # the graph steps X in place through its data pointer; a
+            # contiguous copy would be stepped instead of the caller's tensor
+            raise ValueError("a graph's states must be a contiguous tensor")
         self.X = plan._state(X)
         B = self.X.shape[0]
         self.ra, self.ras = plan._params(Ra, B), plan._params(Ra_s, B)

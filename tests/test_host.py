@@ -69,3 +69,27 @@ def test_build_is_importable_without_gpu():
     import importlib
     m = importlib.import_module("spectraldoublediffusiveconvection_amd")
     assert os.path.exists(m.LIB_PATH)
+
+
+def test_output_tensor_checks():
+    """Plan.step / step_linearized write B*3N doubles through out's data
+    pointer: a wrong dtype, size, device or a strided out is refused before any
+    kernel runs (device.check_out, host logic only)."""
+    torch = pytest.importorskip("torch")
+    from spectraldoublediffusiveconvection_amd import device as dev
+    X = torch.zeros((2, 12), dtype=torch.float64)
+    assert dev.check_out(None, X).shape == X.shape
+    ok = torch.empty((2, 12), dtype=torch.float64)
+    assert dev.check_out(ok, X) is ok
+    assert dev.check_out(torch.empty(24, dtype=torch.float64), X).numel() == 24
+    bad = {
+        "float32": torch.empty((2, 12), dtype=torch.float32),
+        "smaller": torch.empty((1, 12), dtype=torch.float64),
+        "strided": torch.empty((12, 2), dtype=torch.float64).t(),
+        "meta": torch.empty((2, 12), dtype=torch.float64, device="meta"),
+        "not a tensor": np.zeros((2, 12)),
+    }
+    for name, out in bad.items():
+        with pytest.raises(ValueError):
+            dev.check_out(out, X)
+            pytest.fail(name)
