@@ -45,8 +45,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=100.0)
-    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the mixed-vs-float64 run in convecting flow (profiler runs)")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     return ap.parse_args()
 
 
@@ -94,41 +95,111 @@ def sweep_c4(tuning_table, n=256, graph_steps=40, launches=2):
             "ms_per_sweep_step": 1e3 * el / (graph_steps * launches), "states_finite": finite}
 
 
-_WIDE_GAP = {}
+HEADLINE_KEY = (512, 40, 0.35, 5e-3, 1.0, 1.0 / 15.0, False)
+_KEEP = ("rate_jobs_per_s", "submitted", "jobs", "jobs_per_s", "latency_p50_ms",
+         "latency_p99_ms", "instance_steps_per_s")
 
 
-def serving_run(rate, seconds, tuning_table):
-    """Poisson stream of mixed jobs through the serving engine: 60% time
-    integrations (500 steps, sizes as in the sweep), 30% Newton solves and 10%
-    20-point pseudo-arclength continuations in Ra_T of steady wide-gap
-    convection (d=2, Pr=10, N_r=16, N_fm=32, started near the steady state)."""
-    import torch
-
-    from spectraldoublediffusiveconvection_amd import device as dev
+def serving_headline(tuning_table, rates=(10.0, 25.0, 100.0, 200.0), seconds=3.0,
+                     saturation_jobs=1024, capacity=256):
+    """The serving engine on a stream of N_r=40, N_fm=512 time integrations
+    (500 steps each, mixed precision, Ra_T in [2500, 3000], Ra_s in {300, 500},
+    seeded states of norm 1e-3): Poisson arrivals at each rate for `seconds`,
+    then `saturation_jobs` jobs that all arrive at once (the engine's
+    throughput with every slot busy).  Graphs are captured before the streams."""
     from spectraldoublediffusiveconvection_amd import serving
-    Kw, Nrw, d = 32, 16, 2.0
-    if "X" not in _WIDE_GAP:   # steady-state start of the Newton and continuation jobs
-        tsp = dev.Plan.build(Kw, Nrw, d, 0.075, 10.0, 1.0, symmetric=True)
-        x0 = np.random.default_rng(0).random(3 * Kw * (Nrw - 1))
-        _WIDE_GAP["X"] = tsp.step(torch.as_tensor(1e-3 * x0 / np.linalg.norm(x0), device="cuda"),
-                                  6780.0, 0.0, nsteps=26667)[0].cpu().numpy()
-    X = _WIDE_GAP["X"]
-    case = dict(N_fm=Kw, N_r=Nrw, d=d, Pr=10.0, Tau=1.0, Ra=6780.0, Ra_s=0.0, X=X,
-                symmetric=True, restart=100, dRa=10.0)
-    w = WORKLOAD
-    jobs = serving.poisson_stream(rate, seconds, seed=7, newton_case=case,
-                                  ra=(w["Ra_lo"], w["Ra_hi"]), arclength=True)
-    eng = serving.Engine(precision="mixed", capacity=64, chunk=25, newton_workers=3,
+    eng = serving.Engine(precision="mixed", capacity=capacity, chunk=50, newton_workers=0,
                          tuning_table=tuning_table)
     try:
-        eng.run(jobs, realtime=True, max_seconds=seconds * 10 + 30)
-    finally:   # the worker threads end before the interpreter, also on an error
+        eng.prepare(keys=[HEADLINE_KEY])
+        out = {"capacity": capacity, "chunk": 50, "job_steps": 500,
+               "launch_shape": eng.classes[HEADLINE_KEY].plan.tuning(), "rates": []}
+        for rate in rates:
+            jobs = serving.poisson_stream(rate, seconds, seed=int(rate), mix=(1.0, 0.0, 0.0),
+                                          sizes=[(40, 512)])
+            st = _run_engine(eng, jobs, seconds * 10 + 30)
+            st.update({"rate_jobs_per_s": rate, "submitted": len(jobs)})
+            out["rates"].append({k: st[k] for k in _KEEP})
+        rng = np.random.default_rng(3)
+        n3 = 3 * 512 * 39
+        jobs = []
+        for _ in range(saturation_jobs):
+            x = rng.standard_normal(n3)
+            jobs.append(serving.Job("time", 512, 40, float(rng.uniform(2500.0, 3000.0)),
+                                    float(rng.choice((300.0, 500.0))),
+                                    1e-3 * x / np.linalg.norm(x), nsteps=500, arrival=0.0))
+        st = _run_engine(eng, jobs, 120)
+        out["saturation"] = {"jobs": st["jobs"], "elapsed_s": st["elapsed_s"],
+                             "instance_steps_per_s": st["instance_steps_per_s"],
+                             "jobs_per_s": st["jobs_per_s"],
+                             "latency_p50_ms": st["latency_p50_ms"],
+                             "latency_p99_ms": st["latency_p99_ms"]}
+    finally:
         eng.shutdown()
+    return out
+
+
+def _run_engine(eng, jobs, max_seconds):
+    """One stream through an engine that stays up between streams."""
+    eng.done, eng.cancelled = [], []
+    before = eng.instance_steps
+    eng.run(jobs, realtime=True, max_seconds=max_seconds)
     st = eng.stats()
-    st.update({"rate_jobs_per_s": rate, "arrival_window_s": seconds, "submitted": len(jobs),
-               "mix": "60% time (500 steps), 30% newton, "
-                      "10% pseudo-arclength continuation (20 points)"})
+    st["instance_steps_per_s"] = (eng.instance_steps - before) / eng.elapsed
     return st
+
+
+def newton_cases(dev, torch):
+    """Steady thin-gap convection (d=0.353, Ra_T=2360, Ra_S=0, Pr=1, Tau=1,
+    equatorially symmetric) at two of the sweep's resolutions, the starting
+    states of the serving stream's Newton and continuation jobs.  The case is
+    close to onset, where the steady amplitude moves fast with Ra_T: a job's
+    Ra_T is within 0.02% of 2360 and its start the steady state perturbed by
+    0.1% (Newton at dt = 1 needs the reference's limit of 5 iterations once
+    Ra_T moves by 0.04% from a 0.5% perturbed start)."""
+    c = THIN_GAP
+    cases = []
+    for K, Nr in ((64, 20), (128, 30)):
+        X, _ = convecting_state(dev, torch, K, Nr)
+        cases.append(dict(N_fm=K, N_r=Nr, d=c["d"], Pr=c["Pr"], Tau=c["Tau"], Ra=c["Ra"],
+                          Ra_s=c["Ra_s"], X=X[0].cpu().numpy(), symmetric=True, restart=100,
+                          dRa=10.0, ra_spread=2e-4, perturb=1e-3))
+    return cases
+
+
+def serving_mixed(tuning_table, cases, rates=(25.0, 100.0, 200.0), seconds=3.0):
+    """The C5 stream: Poisson arrivals of 60% time integrations (500 steps,
+    sizes drawn from the sweep's twelve classes), 30% Newton solves and 10%
+    20-point pseudo-arclength continuations in Ra_T, the last two at
+    (N_r, N_fm) = (20, 64) and (30, 128) from steady thin-gap convection."""
+    from spectraldoublediffusiveconvection_amd import serving
+    w = WORKLOAD
+    keys = [(k, r, w["d"], w["dt"], w["Pr"], w["Tau"], False)
+            for r in (20, 30, 40) for k in (64, 128, 256, 512)]
+    eng = serving.Engine(precision="mixed", capacity=64, chunk=50, newton_workers=8,
+                         tuning_table=tuning_table)
+    res = []
+    try:
+        probe = serving.poisson_stream(1000.0, 1.0, seed=1, newton_case=cases, arclength=True)
+        eng.prepare(keys=keys, newton_jobs=[j for j in probe if j.kind != "time"])
+        for rate in rates:
+            jobs = serving.poisson_stream(rate, seconds, seed=7, newton_case=cases,
+                                          ra=(w["Ra_lo"], w["Ra_hi"]), arclength=True)
+            st = _run_engine(eng, jobs, seconds * 10 + 30)
+            newton = [j for j in jobs if j.kind == "newton" and j.t_done is not None]
+            cont = [j for j in jobs if j.kind == "continuation" and j.t_done is not None]
+            st.update({"rate_jobs_per_s": rate, "submitted": len(jobs),
+                       "newton_converged": sum(bool(isinstance(j.info, dict)
+                                                    and j.info.get("converged")) for j in newton),
+                       "newton_done": len(newton),
+                       "continuation_points": sum(len(j.result or []) for j in cont),
+                       "continuation_done": len(cont),
+                       "mix": "60% time (500 steps), 30% newton, 10% pseudo-arclength "
+                              "continuation (20 points)"})
+            res.append(st)
+    finally:
+        eng.shutdown()
+    return res
 
 
 def _cpu_model():
@@ -342,20 +413,18 @@ def main():
         "speedup_vs_plain_fp64": value / plain,
         "batched_fp64_graph": batched_fp64,
         "mixed_over_batched_fp64": value / batched_fp64["instance_steps_per_s"],
-        "mixed_vs_fp64": mixed_vs_fp64(dev, torch),
+        "mixed_vs_fp64": None if args.no_accuracy else mixed_vs_fp64(dev, torch),
         "roofline": roof,
         "roofline_full_batch": roof_full,
     }
     if not args.no_sweep:
         line["sweep_mixed_sizes"] = sweep_c4(table)
     if not args.no_serving:
-        line["serving"] = serving_run(args.serve_rate, args.serve_seconds, table)
-        line["serving_rates"] = []
-        for rate in (25.0, 200.0):
-            st = serving_run(rate, 3.0, table)
-            line["serving_rates"].append({k: st[k] for k in (
-                "rate_jobs_per_s", "submitted", "jobs", "jobs_per_s", "latency_p50_ms",
-                "latency_p99_ms", "instance_steps_per_s")})
+        sh = serving_headline(table, seconds=args.serve_seconds)
+        sh["saturation"]["speedup_vs_plain_fp64"] = sh["saturation"]["instance_steps_per_s"] / plain
+        line["serving_40_512"] = sh
+        line["serving_mixed"] = serving_mixed(table, newton_cases(dev, torch),
+                                              seconds=args.serve_seconds)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line))

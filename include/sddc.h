@@ -6,16 +6,20 @@
  * void*; NULL = the default stream) and are stream-ordered.  They return an
  * int status (SDDC_OK = 0); sddc_last_error() gives the message of the last
  * failure on the calling thread.  No entry point allocates device memory
- * except sddc_plan_create / sddc_plan_reserve, sddc_krylov_create and the
- * first call of a transform at a new length (its twiddle tables are cached).
+ * except sddc_plan_create / sddc_plan_reserve, sddc_krylov_create,
+ * sddc_newton_batch_create and the first call of a transform at a new
+ * length (its twiddle tables are cached).
  *
  * Threads: one device-state lock (process-wide) is held for the whole of a
  * graph capture (sddc_graph_create*) and by every call that allocates,
- * frees, copies synchronously or synchronises the device (plan and Krylov
- * create/reserve/destroy, graph destroy, new transform lengths), so such a
- * call on another thread waits for a capture to finish instead of
- * invalidating it.  Compute calls take no lock; calls on one plan (or one
- * Krylov object) must not overlap in time on different threads.
+ * frees, copies synchronously or synchronises the device (plan, Krylov and
+ * batched-Newton create/reserve/destroy, graph destroy, new transform
+ * lengths), so such a call on another thread waits for a capture to finish
+ * instead of invalidating it.  Compute calls take no lock, and on MI355X
+ * kernel launches of another thread (on any stream) during a capture can
+ * invalidate it: callers keep compute calls of other threads off the device
+ * while sddc_graph_create* runs.  Calls on one plan (or one Krylov object)
+ * must not overlap in time on different threads.
  *
  * Batches: a plan describes one size class (N_fm, N_r, d, dt, Pr, Tau); B
  * instances of that class are stored back to back, each in the reference's
@@ -194,7 +198,9 @@ typedef struct sddc_krylov_opts {
   int max_restarts;       /* GMRES cycles (lgmres maxiter, Main.py:534: 250)          */
   int max_newton;         /* Newton iterations (Main.py:518: 5)                       */
   int finite_difference;  /* 1: J v by finite differences of the step, 0: PDFX        */
-  int reserved[5];
+  int check_every;        /* sddc_newton_batch: Arnoldi steps between reads of the    */
+                          /* instances' convergence flags (0: 4)                      */
+  int reserved[4];
 } sddc_krylov_opts;
 typedef struct sddc_krylov_info {
   int converged, iterations, restarts;
@@ -213,6 +219,27 @@ int sddc_gmres(sddc_krylov* k, const double* Ra, const double* Ra_s, const doubl
 /* Newton iteration on Step(X) - X = 0 in place (X is overwritten). */
 int sddc_newton(sddc_krylov* k, const double* Ra, const double* Ra_s, double* X,
                 const sddc_krylov_opts* opts, sddc_newton_info* info, void* stream);
+
+/* Batched Newton-Krylov: _Newton (Main.py:430-554) for B <= max_batch
+ * instances of the plan's size class at once (Ra, Ra_s: device arrays of B
+ * values; X: [B][3N], overwritten; info: B host structs).  Each instance runs
+ * sddc_newton's algorithm (tol_newton, tol_gmres, max_newton, max_restarts,
+ * finite_difference of opts), in lockstep with the others: one J v is one
+ * batched launch of the step kernels, and the Hessenberg least-squares
+ * update (Givens rotations, residual estimate, back substitution) runs on
+ * the device, so the host reads B flags only every opts->check_every
+ * Arnoldi steps, per restart and per Newton iteration.  Converged instances
+ * are frozen while the others iterate, so each instance's result is the same
+ * bits as the same call with that instance alone (B = 1).  The object holds
+ * (restart + 1) * max_batch * 3N doubles of basis and reserves the plan's
+ * workspace for max_batch instances. */
+typedef struct sddc_newton_batch sddc_newton_batch;
+int sddc_newton_batch_create(sddc_plan* plan, int restart, int max_batch,
+                             sddc_newton_batch** out);
+int sddc_newton_batch_destroy(sddc_newton_batch* k);
+int sddc_newton_batched(sddc_newton_batch* k, int B, const double* Ra, const double* Ra_s,
+                      double* X, const sddc_krylov_opts* opts, sddc_newton_info* info,
+                      void* stream);
 
 /* One pseudo-arclength continuation step in Ra_T (_ContinC, Main.py:754-950)
  * on the device.  Y = (X, Ra_T) and Y_dot are device vectors of 3N+1 values,

@@ -45,7 +45,8 @@ class SddcKrylovOpts(ctypes.Structure):
     _fields_ = [("tol_newton", ctypes.c_double), ("tol_gmres", ctypes.c_double),
                 ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
                 ("max_restarts", ctypes.c_int), ("max_newton", ctypes.c_int),
-                ("finite_difference", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("finite_difference", ctypes.c_int), ("check_every", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class SddcKrylovInfo(ctypes.Structure):
@@ -108,6 +109,10 @@ _SIGS = {
                         ctypes.POINTER(SddcKrylovInfo), _V]),
     "sddc_newton": (_I, [_V, _V, _V, _V, ctypes.POINTER(SddcKrylovOpts),
                          ctypes.POINTER(SddcNewtonInfo), _V]),
+    "sddc_newton_batch_create": (_I, [_V, _I, _I, ctypes.POINTER(_V)]),
+    "sddc_newton_batch_destroy": (_I, [_V]),
+    "sddc_newton_batched": (_I, [_V, _I, _V, _V, _V, ctypes.POINTER(SddcKrylovOpts),
+                               ctypes.POINTER(SddcNewtonInfo), _V]),
     "sddc_arclength_step": (_I, [_V, _V, _V, _V, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(SddcKrylovOpts),
                                  ctypes.POINTER(SddcArclengthInfo), _V]),

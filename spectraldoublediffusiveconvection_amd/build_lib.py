@@ -20,7 +20,7 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libsddc.so")
 SOURCES = ["sddc_api.hip", "sddc_transform.hip", "sddc_spectral.hip", "sddc_solve.hip",
-           "sddc_krylov.hip"]
+           "sddc_krylov.hip", "sddc_newton_batch.hip"]
 ARCH = os.environ.get("SDDC_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -689,6 +689,9 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
   }
   hipError_t e = hipStreamEndCapture(s, &g->graph);
   rc = rc2 ? rc2 : hip_check(e, "end capture");
+  // a failed capture leaves its error as the thread's last error, which the
+  // next launch check of this thread would report: clear it here
+  if (rc) (void)hipGetLastError();
   if (!rc) rc = hip_check(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "instantiate");
   if (rc) return bail(rc);
   for (int c = 0; c < nplans; ++c) {

@@ -64,3 +64,19 @@ def start_state(K, Nr, fixture=None):
     """The cached steady state moved to (K, Nr)."""
     z = fixture if fixture is not None else load()
     return cheb_resample(z["X24"], *FIXTURE_SIZE, K, Nr)
+
+
+def steady_on_device(K, Nr, steps=20000):
+    """The cached steady state moved to (K, Nr) and stepped there to steady
+    on the GPU (dt = 0.075, symmetric, as the reference runs the case; the
+    fixed point of the step does not depend on dt).  Returns a numpy state."""
+    import torch
+
+    from spectraldoublediffusiveconvection_amd import device as dev
+    c = THIN_GAP
+    sp = dev.Plan.build(K, Nr, c["d"], 0.075, c["Pr"], c["Tau"], symmetric=True)
+    X = sp.step(torch.as_tensor(start_state(K, Nr), device="cuda"), c["Ra"], c["Ra_s"],
+                nsteps=steps)
+    out = X[0].cpu().numpy()
+    sp.close()
+    return out

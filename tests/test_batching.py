@@ -104,3 +104,59 @@ def test_sub_batch_graph_same_bits(sub):
         torch.cuda.synchronize()
         assert torch.equal(xg, eager), rep
     g.close()
+
+
+@pytest.mark.gpu
+def test_c4_sweep_256_instances_twelve_classes():
+    """The C4 configuration: 256 instances drawn from N_r in {20, 30, 40} x
+    N_fm in {64, 128, 256, 512} (twelve size classes), Ra_T in [2500, 3000],
+    Ra_S in {300, 500}, mixed precision with the autotuned launch shapes of
+    the shipped table, three steps from one graph launch.  For three
+    instances of every class: the oracle's three float64 steps within 2e-5
+    (mixed) and the same sweep in float64 within 1e-12; and each instance
+    alone in its own plan with its class's launch shape gives the same bits
+    as in the graph."""
+    import torch
+
+    from oracle import sddc_oracle as O
+    from spectraldoublediffusiveconvection_amd import batching, device as dev, operators as ops
+    insts = batching.random_sweep(256, seed=2024)
+    table = autotune.TuningTable()
+    sw = batching.Sweep(insts, precision="mixed", tuning_table=table)
+    assert len(sw.classes) == 12 and sw.n_instances == 256
+    g = sw.graph(nsteps=3)
+    g.launch()
+    got = sw.states()
+    g.close()
+    s64 = batching.Sweep(insts, precision="fp64", tuning_table=table)
+    g64 = s64.graph(nsteps=3)
+    g64.launch()
+    got64 = s64.states()
+    g64.close()
+    picked = {}
+    for idx, inst in enumerate(insts):
+        picked.setdefault(batching.class_key(inst), []).append(idx)
+    worst_mixed, worst64 = 0.0, 0.0
+    for key, idxs in picked.items():
+        c = sw.classes[key]
+        shape = c.plan.tuning()
+        tuned = table.lookup(key, c.B, "mixed")
+        assert tuned is not None and shape == tuned, (key, shape, tuned)   # autotuned shape in use
+        o = O.build_operators(*key)
+        for idx in idxs[:3]:
+            inst = insts[idx]
+            x = inst["X0"]
+            for _ in range(3):
+                x = O.step(x, o, inst["Ra"], inst["Ra_s"])
+            e32 = np.linalg.norm(got[idx] - x) / np.linalg.norm(x)
+            e64 = np.linalg.norm(got64[idx] - x) / np.linalg.norm(x)
+            worst_mixed, worst64 = max(worst_mixed, e32), max(worst64, e64)
+            assert e32 < 2e-5, (key, idx, e32)
+            assert e64 < 1e-12, (key, idx, e64)
+            alone = dev.Plan(ops.build(*key), precision="mixed", tuning=shape)
+            y = alone.step(torch.as_tensor(inst["X0"], device="cuda"), inst["Ra"], inst["Ra_s"],
+                           nsteps=3)
+            assert np.array_equal(y.cpu().numpy()[0], got[idx]), (key, idx)
+            alone.close()
+    print("C4 sweep: 12 classes, worst relative error after 3 steps vs oracle: mixed %.2e, "
+          "float64 %.2e" % (worst_mixed, worst64))

@@ -270,3 +270,34 @@ def test_newton_dropin_steady_convection(mods):
     assert ref[-1] is True
     assert rel(Xs, ref[0]) < 1e-8
     assert abs(out[2] / ref[2] - 1) < 1e-8 and abs(out[3] / ref[3] - 1) < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("K_o,K_n", [(16, 32), (32, 16), (48, 64), (64, 48)])
+def test_interp_thetas_vs_oracle_transforms(mods, K_o, K_n):
+    """INTERP_THETAS (device transforms) against the oracle's transforms
+    composed as Matrix_Operators.py:944-1011 composes them: fields laid out
+    on max(K_o, K_n) columns, IDST/IDCT onto that grid, DST/DCT truncated to
+    K_n modes.  Plus: a band-limited state that is carried up and back down
+    comes back, every row but psi's first: as in the reference, psi's rows
+    enter the sine transforms without the shift into the sinusoid index, so
+    row 0 sits in the sin(0 theta) slot and is dropped."""
+    MO = mods["Matrix_Operators"]
+    nr = 7
+    X_o = np.random.default_rng(K_o + K_n).standard_normal(3 * nr * K_o)
+    got = MO.INTERP_THETAS(K_n, K_o, X_o)
+    width = max(K_o, K_n)
+    F = [np.zeros((nr, width)) for _ in range(3)]
+    for f in range(3):
+        F[f][:, :K_o] = X_o[f * K_o * nr:(f + 1) * K_o * nr].reshape(K_o, nr).T
+    hats = (O.dst(O.idst(F[0]), n=K_n), O.dct(O.idct(F[1]), n=K_n), O.dct(O.idct(F[2]), n=K_n))
+    want = np.concatenate([h.T.ravel() for h in hats])
+    assert got.shape == want.shape
+    assert rel(got, want) < 1e-14
+    assert MO.INTERP_THETAS(K_o, K_o, X_o) is X_o
+    if K_n > K_o:
+        back = MO.INTERP_THETAS(K_o, K_n, got)
+        keep = np.ones(3 * nr * K_o, dtype=bool)
+        keep[:nr] = False
+        assert rel(back[keep], X_o[keep]) < 1e-13
+        assert not back[:nr].any()

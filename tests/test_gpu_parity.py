@@ -493,3 +493,23 @@ def test_errors_are_raised():
         p.step(torch.zeros((1, 17), dtype=torch.float64, device="cuda"), 1.0, 1.0)
     with pytest.raises(ValueError):
         p.bsub(torch.zeros((1, 16 * 9), dtype=torch.float64, device="cuda"), 3)
+
+
+@pytest.mark.parametrize("K,Nr", [(64, 20), (512, 40)])
+def test_step_linearized_is_the_central_difference_of_the_step(K, Nr):
+    """On the device: (step(X + h v) - step(X - h v)) / (2h) equals
+    step_linearized(v, X) up to rounding (the step is quadratic in X), and
+    step_linearized agrees with the oracle's."""
+    d, dt, Pr, Tau, Ra, Ras = 0.35, 5e-3, 1.0, 1.0 / 15.0, 3000.0, 500.0
+    p = dev.Plan.build(K, Nr, d, dt, Pr, Tau, max_batch=2)
+    rng = np.random.default_rng(K)
+    n3 = 3 * K * (Nr - 1)
+    X = 0.3 * rng.standard_normal(n3)
+    v = rng.standard_normal(n3)
+    lin = H(p.step_linearized(T(v), T(X), Ra, Ras))[0]
+    for h in (1.0, 0.1):
+        both = H(p.step(T(np.stack([X + h * v, X - h * v])), Ra, Ras))
+        cd = (both[0] - both[1]) / (2 * h)
+        assert rel(cd, lin) < 1e-11 / h, h
+    o = O.build_operators(K, Nr, d, dt, Pr, Tau)
+    assert rel(lin, O.step_linearized(v, X, o, Ra, Ras)) < 1e-12

@@ -275,3 +275,26 @@ def test_linear_growth_rate_target():
         lam.append(np.polyfit(t[-1000:], np.log(norms[-1000:]), 1)[0])
     extrap = 2 * lam[1] - lam[0]
     assert abs(extrap / 0.0018195 - 1) < 0.02, lam
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_step_linearized_is_the_central_difference_of_the_step(sym):
+    """The nonlinear term is quadratic, so the step is a quadratic map of X
+    and (step(X + h v) - step(X - h v)) / (2h) equals the linearised step
+    (PDFX + v, Main.py:498-521) exactly, up to rounding, for any h."""
+    K, Nr = 16, 12
+    o = O.build_operators(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0)
+    rng = np.random.default_rng(17)
+    n3 = 3 * K * (Nr - 1)
+    X = 0.3 * rng.standard_normal(n3)
+    v = rng.standard_normal(n3)
+    lin = O.step_linearized(v, X, o, 3000.0, 500.0, symmetric=sym)
+    for h in (1.0, 0.1, 1e-3):
+        cd = (O.step(X + h * v, o, 3000.0, 500.0, symmetric=sym)
+              - O.step(X - h * v, o, 3000.0, 500.0, symmetric=sym)) / (2 * h)
+        assert np.linalg.norm(cd - lin) / np.linalg.norm(lin) < 1e-11 / h, h
+    # a plausible slip (the factor 2 of F(X, v) + F(v, X) lost) fails it
+    half = 0.5 * (lin + O.step_linearized(v, 0 * X, o, 3000.0, 500.0, symmetric=sym))
+    cd = (O.step(X + v, o, 3000.0, 500.0, symmetric=sym)
+          - O.step(X - v, o, 3000.0, 500.0, symmetric=sym)) / 2
+    assert np.linalg.norm(cd - half) / np.linalg.norm(lin) > 1e-6
