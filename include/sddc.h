@@ -158,6 +158,25 @@ int sddc_bsub(sddc_plan* plan, int B, int field, const double* g, double* f, voi
 int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
               const double* X_in, double* X_out, int nsteps, int linear, void* stream);
 
+/* sddc_step for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The step's intermediates (suffix sums,
+ * linear part, slabs, products, right-hand side, solve exit states) live in
+ * one static device array of SDDC_STATIC_SCRATCH_BYTES per GPU, held from
+ * the first plan created on that GPU until the process ends, not in the
+ * plan's workspace; the call makes no hipMalloc, hipFree or other allocating
+ * call, and B is not limited by the plan's reserved batch.  A batch whose
+ * intermediates do not fit the array is refused with SDDC_ERR_INVALID (the
+ * message gives how many instances of the class fit: 25 at N_r = 40,
+ * N_fm = 512 in mixed precision, 23 in float64, with unsegmented chains).
+ * Every call on one GPU uses
+ * the same array: calls that overlap in time on one GPU (other streams,
+ * other threads, other plans) race.  Results are the same bits as
+ * sddc_step. */
+#define SDDC_STATIC_SCRATCH_BYTES (64ll << 20)
+int sddc_step_static(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
+                     const double* X_in, double* X_out, int nsteps, int linear, void* stream);
+
 /* The step linearised about X applied to dv: PDFX without "- dv"
  * (Main.py:498-521, _ContinC Main.py:804-827). */
 int sddc_step_linearized(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,

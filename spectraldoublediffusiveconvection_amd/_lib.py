@@ -94,6 +94,7 @@ _SIGS = {
     "sddc_nlin_dfx": (_I, [_V, _I, _V, _V, _V, _V]),
     "sddc_bsub": (_I, [_V, _I, _I, _V, _V, _V]),
     "sddc_step": (_I, [_V, _I, _V, _V, _V, _V, _I, _I, _V]),
+    "sddc_step_static": (_I, [_V, _I, _V, _V, _V, _V, _I, _I, _V]),
     "sddc_step_linearized": (_I, [_V, _I, _V, _V, _V, _V, _V, _V]),
     "sddc_diagnostics": (_I, [_V, _I, _V, _V, _V]),
     "sddc_graph_create": (_I, [_V, _I, _V, _V, _V, _I, _V, ctypes.POINTER(_V)]),

@@ -208,6 +208,8 @@ using namespace sddc;
 
 extern "C" {
 
+static int static_scratch(int dev, unsigned char** out);
+
 const char* sddc_last_error(void) { return g_last_error.c_str(); }
 const char* sddc_version(void) { return "sddc 0.1 gfx950"; }
 
@@ -263,6 +265,11 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
     return code;
   };
   if (hipGetDevice(&p.device) != hipSuccess) return bail(fail(SDDC_ERR_HIP, "no HIP device"));
+  // set-up of sddc_step_static: this device's address of the static scratch
+  {
+    unsigned char* base = nullptr;
+    if ((rc = static_scratch(p.device, &base))) return bail(rc);
+  }
   int lds_max = 0;
   (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, p.device);
   if ((size_t)nonlinear_lds_bytes(p, precision, 0) > (size_t)(lds_max > 0 ? lds_max : 65536))
@@ -539,6 +546,68 @@ static Plan sub_plan(const Plan& p, int b0, int nb) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free step (sddc_step_static): the step's intermediates in one
+// static device array per GPU instead of the plan's workspace.  Every buffer
+// carved from it is written before it is read within a call (suffix sums,
+// linear part, slabs, products, right-hand side, segment exit states; the
+// solve's sink is only written), so nothing in it has to start at zero and
+// a call does not depend on what an earlier call left there.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticScratchBytes = (size_t)SDDC_STATIC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<std::pair<int, unsigned char*>> g_scratch_ptr;   // (device, address)
+
+// The address of this device's copy of `scratch`, looked up once per device
+// with that device current (hipGetSymbolAddress); the array's name is a host
+// stand-in and is never passed to a kernel or copy.
+static int static_scratch(int dev, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  for (auto& e : g_scratch_ptr)
+    if (e.first == dev) {
+      *out = e.second;
+      return SDDC_OK;
+    }
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return hip_check(e != hipSuccess ? e : hipErrorUnknown, "hipGetSymbolAddress(scratch)");
+  g_scratch_ptr.push_back({dev, static_cast<unsigned char*>(ptr)});
+  *out = static_cast<unsigned char*>(ptr);
+  return SDDC_OK;
+}
+
+// The plan restricted to B instances with its step workspace carved from
+// `base` (256-byte aligned pieces); *need = the bytes the batch uses.
+static Plan static_view(const Plan& p, int B, unsigned char* base, size_t* need) {
+  Plan v = p;
+  const size_t N = (size_t)p.K * p.nr;
+  const size_t rs = p.precision == SDDC_MIXED ? sizeof(float) : sizeof(double);
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* q = base + off;
+    off += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  v.P = reinterpret_cast<double*>(take((size_t)B * N * 8));
+  v.L = reinterpret_cast<double*>(take((size_t)B * 3 * N * 8));
+  v.G = reinterpret_cast<double*>(take((size_t)B * 3 * N * 8));
+  v.Xa = p.symmetric ? reinterpret_cast<double*>(take((size_t)B * 3 * N * 8)) : nullptr;
+  v.slabs = take((size_t)B * 7 * N * 8);   // sized as the plan's workspace sizes it
+  v.F = take((size_t)B * 4 * N * rs);
+  v.solve_scratch = reinterpret_cast<double*>(take((4 * N + 256) * 8));
+  const int exb = ((B + 3) / 4) * 4;
+  v.chain_EX = reinterpret_cast<double*>(take((size_t)exb * 6 * p.nseg * 2 * p.NRP * 8));
+  v.ex_batch = exb;
+  v.Xb = v.Pa = nullptr;
+  v.slabs2 = nullptr;
+  v.rowval = v.params = nullptr;
+  v.max_batch = B;
+  *need = off;
+  return v;
+}
+
 // Parts [b0, b0 + nb) of a batch of B for `parts` forked streams: sizes are
 // multiples of 16 (the last part takes the rest); fewer parts when B is small.
 static std::vector<std::pair<int, int>> split_batch(int B, int parts) {
@@ -561,6 +630,32 @@ int sddc_step(sddc_plan* plan, int B, const double* Ra, const double* Ra_s, cons
   if (rc) return rc;
   if (nsteps < 0) return fail(SDDC_ERR_INVALID, "negative step count");
   return run_steps(p, B, Ra, Ra_s, X_in, X_out, nsteps, linear, (hipStream_t)stream);
+}
+
+int sddc_step_static(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
+                     const double* X_in, double* X_out, int nsteps, int linear, void* stream) {
+  if (!plan) return fail(SDDC_ERR_INVALID, "null plan");
+  Plan& p = plan->p;
+  if (B < 0) return fail(SDDC_ERR_INVALID, "negative batch");
+  if (nsteps < 0) return fail(SDDC_ERR_INVALID, "negative step count");
+  int dev = 0;
+  SDDC_HIP(hipGetDevice(&dev));
+  if (dev != p.device) return fail(SDDC_ERR_INVALID, "the plan was created on another device");
+  unsigned char* base = nullptr;
+  if (int rc = static_scratch(dev, &base)) return rc;
+  size_t need = 0;
+  Plan v = static_view(p, B, base, &need);
+  if (need > kStaticScratchBytes) {
+    size_t one = 0;
+    (void)static_view(p, 1, base, &one);
+    char buf[200];
+    snprintf(buf, sizeof(buf),
+             "batch of %d needs %zu bytes of static scratch, above its %zu (about %zu instances "
+             "of this size class fit)",
+             B, need, kStaticScratchBytes, one ? kStaticScratchBytes / one : (size_t)0);
+    return fail(SDDC_ERR_INVALID, buf);
+  }
+  return run_steps(v, B, Ra, Ra_s, X_in, X_out, nsteps, linear, (hipStream_t)stream);
 }
 
 int sddc_step_linearized(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,

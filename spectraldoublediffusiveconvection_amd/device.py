@@ -172,6 +172,27 @@ class Plan:
                                         _dev_ptr(out), int(nsteps), int(bool(linear)), _stream()))
         return out
 
+    def step_static(self, X, Ra, Ra_s, out, nsteps=1, linear=False):
+        """step() without the plan's workspace (sddc_step_static): the
+        intermediates live in the library's static per-GPU scratch, so the
+        call allocates no device memory and the batch may exceed the plan's
+        reserved size.  X, out: contiguous (B, 3N) float64 tensors on the
+        plan's device; Ra, Ra_s: device tensors of length B (host values
+        would be copied into new tensors).  Calls overlapping on one GPU race."""
+        if not (isinstance(X, torch.Tensor) and X.is_cuda and X.dtype == torch.float64
+                and X.is_contiguous() and X.device == self.device):
+            raise ValueError("X must be a contiguous float64 tensor on the plan's device")
+        X = X.view(1, -1) if X.dim() == 1 else X
+        if X.dim() != 2 or X.shape[-1] != 3 * self.N:
+            raise ValueError("states must be (B, 3N) with 3N = %d" % (3 * self.N))
+        B = X.shape[0]
+        out = check_out(out, X)
+        ra, ras = self._params(Ra, B), self._params(Ra_s, B)
+        _lib.check(_lib.lib().sddc_step_static(self._h, B, _dev_ptr(ra), _dev_ptr(ras),
+                                               _dev_ptr(X), _dev_ptr(out), int(nsteps),
+                                               int(bool(linear)), _stream()))
+        return out
+
     def step_linearized(self, dv, X, Ra, Ra_s, out=None):
         """PDFX without '- dv' (Main.py:498-521)."""
         X = self._state(X)
