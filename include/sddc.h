@@ -74,6 +74,13 @@ typedef struct sddc_operators {
   const double* Linv_A4;    /* N_fm*nr^2  A4_TSTEP_MATS(Pr*dt) (Matrix_Operators.py:1089-1112) */
   const double* Linv_T;     /* N_fm*nr^2  NAB2_TSTEP_MATS(dt)  (Matrix_Operators.py:1014-1030) */
   const double* Linv_S;     /* N_fm*nr^2  NAB2_TSTEP_MATS(Tau*dt)                              */
+  /* Optional (NULL: the NAB2 solves use the per-mode blocks above).  The
+   * eigenbasis of P = r^2 - h r^2 Lap, whose shifts P + h j(j+1) I are the
+   * NAB2 matrices of mode j (h = dt for T, Tau*dt for S): nr eigenvalues lam,
+   * then V and V^-1 (nr x nr, row-major) with P = V diag(lam) V^-1.  With
+   * them each NAB2 chain decouples into nr scalar recurrences. */
+  const double* modes_T;    /* nr + 2 nr^2, h = dt      */
+  const double* modes_S;    /* nr + 2 nr^2, h = Tau*dt  */
 } sddc_operators;
 
 typedef struct sddc_plan sddc_plan;
@@ -86,20 +93,31 @@ typedef struct sddc_plan sddc_plan;
  *                    kernels), 3 (quad-lane, 2 per A4 workgroup and 4 per
  *                    NAB2 workgroup), 5 (as 3, the A4 chains with two
  *                    exchanges per row and one streamed block), 6 (as 5,
- *                    3 per A4 workgroup), 16 (MFMA kernel).  1-4 give the
- *                    same bits, as do 5 and 6; 5, 6 and 16 sum in another
- *                    order than 1-4 (equal to rounding), so the choice is
+ *                    3 per A4 workgroup), 7 and 8 (as 5 with four and six
+ *                    rows of operator data in flight), 9 (one instance per
+ *                    A4 workgroup, two exchanges, six rows in flight), 10
+ *                    (one instance per A4 workgroup, one exchange, four
+ *                    rows in flight), 16 (MFMA kernel).  1-4 and 10 give the
+ *                    same bits, as do 5-9; these two groups and 16 sum in
+ *                    different orders (equal to rounding), so the choice is
  *                    fixed per plan
  *   sub_batches      graphs only: the batch is split into this many parts
  *                    (1-8), each stepped on its own forked stream inside the
  *                    graph, so that the latency-bound solves of one part
  *                    overlap the transforms of another.  Results are the
- *                    same bits as one batch. */
+ *                    same bits as one batch.
+ *   nab2_solve       the T and S solves: 0 or 1 chains of per-mode block
+ *                    inverses (the reference's form), 2 scalar recurrences
+ *                    in the eigenbasis of the operator set's modes_T /
+ *                    modes_S (equal to rounding, about 1e-14 relative; a
+ *                    separate launch after the A4 chains, about three times
+ *                    faster than the block chains on its own). */
 typedef struct sddc_tuning {
   int chain_segment;
   int solve_instances;
   int sub_batches;
-  int reserved[5];
+  int nab2_solve;
+  int reserved[4];
 } sddc_tuning;
 
 /* Plan lifecycle.  Replaces the operator tuple of Build_Matrix_Operators

@@ -33,12 +33,14 @@ class SddcOperators(ctypes.Structure):
         ("IR2v", _dp), ("IR4v", _dp), ("DT0", _dp), ("Rsq", _dp), ("GR", _dp),
         ("D2_A4", _dp), ("IR2_A4", _dp), ("IR4_A4", _dp),
         ("Linv_A4", _dp), ("Linv_T", _dp), ("Linv_S", _dp),
+        ("modes_T", _dp), ("modes_S", _dp),
     ]
 
 
 class SddcTuning(ctypes.Structure):
     _fields_ = [("chain_segment", ctypes.c_int), ("solve_instances", ctypes.c_int),
-                ("sub_batches", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("sub_batches", ctypes.c_int), ("nab2_solve", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class SddcKrylovOpts(ctypes.Structure):

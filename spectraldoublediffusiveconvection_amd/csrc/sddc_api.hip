@@ -224,6 +224,7 @@ int sddc_plan_tuning(const sddc_plan* plan, sddc_tuning* out) {
   out->chain_segment = plan->p.seg_m;
   out->solve_instances = plan->p.tune_solve_nbi;
   out->sub_batches = plan->p.tune_sub_batches;
+  out->nab2_solve = plan->p.nab2_modal ? 2 : 1;
   return SDDC_OK;
 }
 
@@ -300,15 +301,22 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
   }
   if ((rc = upload(&p.vecs, v.data(), v.size()))) return bail(rc);
   p.NRS = ((nr + 7) / 8) * 8;
-  if (tuning && tuning->solve_instances != 0 && tuning->solve_instances != 1 &&
-      tuning->solve_instances != 2 && tuning->solve_instances != 3 &&
-      tuning->solve_instances != 4 && tuning->solve_instances != 5 &&
-      tuning->solve_instances != 6 && tuning->solve_instances != 16)
-    return bail(fail(SDDC_ERR_INVALID, "solve_instances must be 0, 1, 2, 3, 4, 5, 6 or 16"));
+  if (tuning && !((tuning->solve_instances >= 0 && tuning->solve_instances <= 10) ||
+                  tuning->solve_instances == 16))
+    return bail(fail(SDDC_ERR_INVALID, "solve_instances must be 0-10 or 16"));
   if (tuning && tuning->chain_segment < 0)
     return bail(fail(SDDC_ERR_INVALID, "chain_segment must be >= 0"));
   if (tuning && (tuning->sub_batches < 0 || tuning->sub_batches > 8))
     return bail(fail(SDDC_ERR_INVALID, "sub_batches must be in [0, 8]"));
+  if (tuning && (tuning->nab2_solve < 0 || tuning->nab2_solve > 2))
+    return bail(fail(SDDC_ERR_INVALID, "nab2_solve must be 0, 1 or 2"));
+  {
+    const bool have_modes = ops->modes_T && ops->modes_S;
+    const int want = tuning ? tuning->nab2_solve : 0;
+    if (want == 2 && !have_modes)
+      return bail(fail(SDDC_ERR_INVALID, "nab2_solve 2 needs the operator set's modes_T and modes_S"));
+    p.nab2_modal = want == 2 ? 1 : 0;
+  }
   p.tune_solve_nbi = tuning ? tuning->solve_instances : 0;
   p.tune_sub_batches = (tuning && tuning->sub_batches > 0) ? tuning->sub_batches : 1;
   if ((rc = build_chain(p, ops, max_batch > 0 ? max_batch : 1,

@@ -116,6 +116,10 @@ struct Plan {
   double* chain_E[3] = {nullptr, nullptr, nullptr};   // segment exit maps (nseg > 1)
   double* chain_Lm[3] = {nullptr, nullptr, nullptr};  // MFMA-packed block inverses
   double* chain_DLm = nullptr;                        // MFMA-packed D2 Linv_j (A4)
+  // NAB2 solves in the eigenbasis of r^2 - h r^2 Lap (fields 1, 2), when set:
+  // per field lam [NRS] (padding 1), V [nr][NRS], V^-1 [NRS][NRS] (padding 0)
+  int nab2_modal = 0;
+  double* nab2_modes[3] = {nullptr, nullptr, nullptr};
   TableSet tabM, tabKE;
   // workspace (device), sized for max_batch instances
   double* P = nullptr;

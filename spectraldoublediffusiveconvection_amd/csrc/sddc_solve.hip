@@ -924,14 +924,186 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_mfma(ChainArgs a
   }
 }
 
+// Chain workgroups along x: (field, chain) pairs; two when only the A4
+// solve runs (the NAB2 fields in the eigenbasis or not asked for).
+static inline unsigned chain_grid_x(const ChainArgs& a) { return a.fields_mask == 1 ? 2u : 6u; }
+
+// ---------------------------------------------------------------------------
+// NAB2 solves in the eigenbasis (Plan::nab2_modal).  The block of row j is
+// Linv_j = (P + h j(j+1) I)^-1 with P = V diag(lam) V^-1, so in the variables
+// gt = V^-1 g, ft = V^-1 f, bt = V^-1 b the chain of NAB2_BSub_TSTEP_V2
+// (Matrix_Operators.py:1061-1083)
+//   f_j = Linv_j (g_j - c_j b),  then b += 2 h j f_j for row j - 2
+// is nr independent scalar recurrences (c_j = 1/2 at j = 0, else 1):
+//   ft_j = (gt_j - c_j bt) q_j,  q_j = 1 / (lam + h j (j+1)),
+//   bt  <- bt + 2 h j ft_j = (lam + h j (j-1)) q_j bt + 2 h j q_j gt_j.
+// One workgroup per (instance, field, chain) takes the chain in chunks of 64
+// consecutive rows from the top.  Per chunk: the rows go to LDS (the next
+// chunk's rows are loaded meanwhile); gt = g V^-T as four 16-row tiles times
+// the wavefront's 16-column tile of V^-T on f64 MFMA (16x16x4: A = rows from
+// LDS, B = V^-1 entries held in registers for the whole chain); the
+// recurrences run on wavefront 0, one component per lane, carrying bt from
+// chunk to chunk in registers; f = ft V^T the same way; the rows go back to
+// memory.
+// ---------------------------------------------------------------------------
+struct ModalArgs {
+  const double* G;
+  double* X;
+  long long gstride, xstride;
+  long long goff[3], xoff[3];
+  int K, nr, symmetric, field0;
+  double h[3];
+};
+
+typedef double v4d_m __attribute__((ext_vector_type(4)));
+
+// out[row][16w + l%16] = sum_e buf[row][e] M[16w + l%16][e] for the 64 rows
+// of the chunk (lane l, register v of tile t: row 16t + l/16 + 4v), with
+// bm[s] = M[16w + l%16][4s + l/16].  All reads of buf happen here; the
+// caller writes the results after a barrier.
+template <int NRS, int PITCH>
+__device__ __forceinline__ void modal_tiles(const double* buf, const double (&bm)[NRS / 4],
+                                            int lane, v4d_m (&acc)[4]) {
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = (v4d_m){0.0, 0.0, 0.0, 0.0};
+  const double* arow = buf + (lane & 15) * PITCH + (lane >> 4);
+#pragma unroll
+  for (int s = 0; s < NRS / 4; ++s)
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[16 * t * PITCH + 4 * s], bm[s], acc[t],
+                                                    0, 0, 0);
+}
+
+template <int NRS, int PITCH>
+__device__ __forceinline__ void modal_store(double* buf, const v4d_m (&acc)[4], int col, int lane) {
+  if (col >= NRS) return;   // columns past NRS do not exist in buf (and are zero)
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) buf[(16 * t + (lane >> 4) + 4 * v) * PITCH + col] = acc[t][v];
+}
+
+template <int NRS, int WR>
+__global__ __launch_bounds__(64 * WR) void k_nab2_modal(ModalArgs a, const double* __restrict__ m1,
+                                                        const double* __restrict__ m2) {
+  constexpr int CH = 64;                  // rows per chunk
+  constexpr int PITCH = NRS + 1;          // odd: column accesses spread over the banks
+  constexpr int RPW = (CH + WR - 1) / WR; // rows of a chunk each wavefront loads
+  __shared__ double buf[CH * PITCH];
+  const int field = a.field0 + (int)(blockIdx.x >> 1), chain = blockIdx.x & 1;
+  const int K = a.K, nr = a.nr;
+  const int start = K - 1 - chain, n = start / 2 + 1;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const double* __restrict__ g = a.G + (size_t)blockIdx.y * a.gstride + a.goff[field];
+  double* __restrict__ x = a.X + (size_t)blockIdx.y * a.xstride + a.xoff[field];
+  if (a.symmetric && chain == 0) {   // Eq_SYM: odd T/S rows are zero (Matrix_Operators.py:1054)
+    for (int i = w; i < n; i += WR)
+      if (lane < nr) x[(size_t)(start - 2 * i) * nr + lane] = 0.0;
+    return;
+  }
+  const double* __restrict__ md = field == 1 ? m1 : m2;
+  const double h = a.h[field];
+  const double lamk = md[lane < NRS ? lane : 0];
+  // B operands of this wavefront's column tile: V^-1 [NRS][NRS] and V [nr][NRS]
+  const int col = 16 * w + (lane & 15);
+  double bvi[NRS / 4], bv[NRS / 4];
+#pragma unroll
+  for (int s = 0; s < NRS / 4; ++s) {
+    const int e = 4 * s + (lane >> 4);
+    bvi[s] = col < NRS ? md[NRS + (size_t)nr * NRS + (size_t)col * NRS + e] : 0.0;
+    bv[s] = col < nr ? md[NRS + (size_t)col * NRS + e] : 0.0;
+  }
+  double pre[RPW];
+  auto fetch = [&](int c0) {
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {
+      const int i = w + WR * u;
+      const bool ok = i < CH && c0 + i < n && lane < nr;
+      pre[u] = ok ? g[(size_t)(start - 2 * (c0 + i)) * nr + lane] : 0.0;
+    }
+  };
+  fetch(0);
+  double bt = 0.0;
+  v4d_m acc[4];
+  for (int c0 = 0; c0 < n; c0 += CH) {
+    const int cn = n - c0 < CH ? n - c0 : CH;
+#pragma unroll
+    for (int u = 0; u < RPW; ++u) {   // padding rows and columns are zero
+      const int i = w + WR * u;
+      if (i < CH && lane < NRS) buf[i * PITCH + lane] = pre[u];
+    }
+    __syncthreads();
+    if (c0 + CH < n) fetch(c0 + CH);
+    modal_tiles<NRS, PITCH>(buf, bvi, lane, acc);
+    __syncthreads();
+    modal_store<NRS, PITCH>(buf, acc, col, lane);
+    __syncthreads();
+    if (w == 0 && lane < nr) {
+      // eight rows at a time: their inputs and q are ready before the
+      // recurrence reaches them (rows past the chunk are zero and skipped)
+      for (int i0 = 0; i0 < cn; i0 += 8) {
+        double gt[8], q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int j = start - 2 * (c0 + i0 + u);
+          gt[u] = buf[(i0 + u) * PITCH + lane];
+          q[u] = 1.0 / (lamk + (h * j) * (j + 1));
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (i0 + u < cn) {
+            const int j = start - 2 * (c0 + i0 + u);
+            const double hj = h * j;
+            const double cj = j == 0 ? 0.5 : 1.0;
+            buf[(i0 + u) * PITCH + lane] = (gt[u] - cj * bt) * q[u];
+            bt = fma((lamk + hj * (j - 1)) * q[u], bt, (2.0 * hj * q[u]) * gt[u]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    modal_tiles<NRS, PITCH>(buf, bv, lane, acc);
+    __syncthreads();
+    modal_store<NRS, PITCH>(buf, acc, col, lane);
+    __syncthreads();
+    for (int i = w; i < cn; i += WR)
+      if (lane < nr) x[(size_t)(start - 2 * (c0 + i)) * nr + lane] = buf[i * PITCH + lane];
+    __syncthreads();
+  }
+}
+
+template <int NRS>
+static int launch_modal_t(const ModalArgs& a, int nf, int B, const Plan& p, hipStream_t s) {
+  using S = CShape<NRS>;
+  const dim3 grid(2 * nf, B), block(64 * S::WR);
+  hipLaunchKernelGGL((k_nab2_modal<NRS, S::WR>), grid, block, 0, s, a, p.nab2_modes[1],
+                     p.nab2_modes[2]);
+  return hip_check(hipGetLastError(), "k_nab2_modal");
+}
+
+static int dispatch_modal(const ModalArgs& a, int nf, int B, const Plan& p, hipStream_t s) {
+  switch (p.NRS) {
+    case 8: return launch_modal_t<8>(a, nf, B, p, s);
+    case 16: return launch_modal_t<16>(a, nf, B, p, s);
+    case 24: return launch_modal_t<24>(a, nf, B, p, s);
+    case 32: return launch_modal_t<32>(a, nf, B, p, s);
+    case 40: return launch_modal_t<40>(a, nf, B, p, s);
+    case 48: return launch_modal_t<48>(a, nf, B, p, s);
+    case 56: return launch_modal_t<56>(a, nf, B, p, s);
+    case 64: return launch_modal_t<64>(a, nf, B, p, s);
+    default: return fail(SDDC_ERR_UNSUPPORTED, "unsupported radial size");
+  }
+}
+
 template <int NRS>
 static int launch_chain_mfma_t(const ChainArgs& a, int B, hipStream_t s) {
   using S = CShape<NRS>;
-  const dim3 grid(6, a.nseg, (B + 15) / 16), block(64 * S::WR);
+  const dim3 grid(chain_grid_x(a), a.nseg, (B + 15) / 16), block(64 * S::WR);
   hipLaunchKernelGGL((k_chain_mfma<NRS>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_mfma")) return rc;
   if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup
-    const dim3 gj(6, a.nseg, (B + 1) / 2);
+    const dim3 gj(chain_grid_x(a), a.nseg, (B + 1) / 2);
     hipLaunchKernelGGL((k_chain_join<NRS, 2>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
   }
@@ -941,7 +1113,7 @@ static int launch_chain_mfma_t(const ChainArgs& a, int B, hipStream_t s) {
 template <int NRS, int NB>
 static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
   using S = CShape<NRS>;
-  const dim3 grid(6, a.nseg, (B + NB - 1) / NB), block(64 * S::WR);
+  const dim3 grid(chain_grid_x(a), a.nseg, (B + NB - 1) / NB), block(64 * S::WR);
   hipLaunchKernelGGL((k_chain_local<NRS, NB>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local")) return rc;
   if (a.nseg > 1) {
@@ -955,11 +1127,11 @@ template <int NRS, int NBA, int NBN, bool X2, int KD>
 static int launch_chain_mixed_t(const ChainArgs& a, int B, hipStream_t s) {
   using S = CShape<NRS>;
   constexpr int NBMIN = NBA < NBN ? NBA : NBN;
-  const dim3 grid(6, a.nseg, (B + NBMIN - 1) / NBMIN), block(64 * S::WR);
+  const dim3 grid(chain_grid_x(a), a.nseg, (B + NBMIN - 1) / NBMIN), block(64 * S::WR);
   hipLaunchKernelGGL((k_chain_local_mixed<NRS, NBA, NBN, X2, KD>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local_mixed")) return rc;
   if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup
-    const dim3 gj(6, a.nseg, (B + 1) / 2);
+    const dim3 gj(chain_grid_x(a), a.nseg, (B + 1) / 2);
     hipLaunchKernelGGL((k_chain_join<NRS, 2>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
   }
@@ -974,6 +1146,10 @@ static int dispatch_nb(const ChainArgs& a, int NB, int B, hipStream_t s) {
     case 3: return launch_chain_mixed_t<NRS, 2, 4, false, 2>(a, B, s);
     case 5: return launch_chain_mixed_t<NRS, 2, 4, true, 3>(a, B, s);
     case 6: return launch_chain_mixed_t<NRS, 3, 4, true, 2>(a, B, s);
+    case 7: return launch_chain_mixed_t<NRS, 2, 4, true, 4>(a, B, s);
+    case 8: return launch_chain_mixed_t<NRS, 2, 4, true, 6>(a, B, s);
+    case 9: return launch_chain_mixed_t<NRS, 1, 4, true, 6>(a, B, s);
+    case 10: return launch_chain_mixed_t<NRS, 1, 4, false, 4>(a, B, s);
     case 4: return launch_chain_t<NRS, 4>(a, B, s);
     default: return fail(SDDC_ERR_INVALID, "unsupported instances per workgroup");
   }
@@ -1046,6 +1222,26 @@ int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
   a.dtf[0] = p.Pr * p.dt;
   a.dtf[1] = p.dt;
   a.dtf[2] = p.Tau * p.dt;   // NAB2 of S uses Tau*dt (Main.py:281)
+  if (p.nab2_modal && (fields_mask & 6)) {
+    ModalArgs m;
+    m.G = G;
+    m.X = X;
+    m.gstride = gstride;
+    m.xstride = xstride;
+    for (int f = 0; f < 3; ++f) {
+      m.goff[f] = goff[f];
+      m.xoff[f] = xoff[f];
+      m.h[f] = a.dtf[f];
+    }
+    m.K = p.K;
+    m.nr = p.nr;
+    m.symmetric = p.symmetric;
+    m.field0 = (fields_mask & 2) ? 1 : 2;
+    const int nf = ((fields_mask >> 1) & 1) + ((fields_mask >> 2) & 1);
+    if (int rc = dispatch_modal(m, nf, B, p, s)) return rc;
+    a.fields_mask = fields_mask & 1;
+    if (!a.fields_mask) return SDDC_OK;
+  }
   int NB = B >= 64 ? 4 : (B >= 8 ? 2 : 1);
   if (p.tune_solve_nbi > 0) NB = p.tune_solve_nbi;   // 16: MFMA chain kernel
   return dispatch_chain(a, p.NRS, NB, B, s);
@@ -1152,11 +1348,25 @@ int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment) 
   p.seg_m = m;
   p.nseg = (n + m - 1) / m;
   const double* inv[3] = {ops->Linv_A4, ops->Linv_T, ops->Linv_S};
+  const int nfb = p.nab2_modal ? 1 : 3;   // fields solved with per-mode blocks
   std::vector<double> buf((size_t)K * BLK);
-  for (int f = 0; f < 3; ++f) {
+  for (int f = 0; f < nfb; ++f) {
     for (int jj = 0; jj < K; ++jj)
       pack_quad(inv[f] + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
     if (int rc = upload_vec(&p.chain_L[f], buf)) return rc;
+  }
+  if (p.nab2_modal) {   // lam | V | V^-1, rows padded to NRS
+    const double* md[3] = {nullptr, ops->modes_T, ops->modes_S};
+    for (int f = 1; f < 3; ++f) {
+      std::vector<double> m((size_t)NRS + (size_t)nr * NRS + (size_t)NRS * NRS, 0.0);
+      for (int k = 0; k < NRS; ++k) m[k] = k < nr ? md[f][k] : 1.0;
+      for (int r = 0; r < nr; ++r)
+        for (int k = 0; k < nr; ++k) {
+          m[NRS + (size_t)r * NRS + k] = md[f][nr + (size_t)r * nr + k];
+          m[NRS + (size_t)nr * NRS + (size_t)r * NRS + k] = md[f][nr + NN + (size_t)r * nr + k];
+        }
+      if (int rc = upload_vec(&p.nab2_modes[f], m)) return rc;
+    }
   }
   // D2 Linv_j per mode (float64 on the host): the A4 chains carry D2 f_e
   std::vector<double> dl((size_t)K * NN);
@@ -1170,7 +1380,7 @@ int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment) 
     if (int rc = upload_vec(&p.chain_D2, d2b)) return rc;
   }
   if (p.tune_solve_nbi == 16) {   // operands of the MFMA chain kernel
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < nfb; ++f) {
       for (int jj = 0; jj < K; ++jj)
         pack_mfma(inv[f] + jj * NN, nr, nr, nr, NRP, NRS, buf.data() + jj * BLK);
       if (int rc = upload_vec(&p.chain_Lm[f], buf)) return rc;
@@ -1185,7 +1395,7 @@ int build_chain(Plan& p, const sddc_operators* ops, int max_batch, int segment) 
   // Q = -c_j Linv_j T, T <- T + 2 dtf j Q; E = T after the segment.
   const double dts[3] = {p.Pr * p.dt, p.dt, p.Tau * p.dt};
   std::vector<double> T(NN), Q(NN), LT(NN);
-  for (int f = 1; f < 3; ++f) {
+  for (int f = 1; f < nfb; ++f) {
     std::vector<double> qpk((size_t)K * BLK, 0.0), epk((size_t)2 * p.nseg * BLK, 0.0);
     for (int chain = 0; chain < 2; ++chain) {
       const int start = K - 1 - chain;
@@ -1274,6 +1484,10 @@ void free_chain(Plan& p) {
   }
   if (p.chain_DLm) (void)hipFree(p.chain_DLm);
   p.chain_DLm = nullptr;
+  for (int f = 0; f < 3; ++f) {
+    if (p.nab2_modes[f]) (void)hipFree(p.nab2_modes[f]);
+    p.nab2_modes[f] = nullptr;
+  }
 }
 
 }  // namespace sddc

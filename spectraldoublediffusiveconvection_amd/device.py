@@ -79,6 +79,9 @@ class Plan:
         for name in ("D", "R", "Dr", "D2s", "D2r2", "IR2v", "IR4v", "DT0", "Rsq", "GR", "D2_A4",
                      "IR2_A4", "IR4_A4", "Linv_A4", "Linv_T", "Linv_S"):
             setattr(s, name, arr(name))
+        for name in ("modes_T", "modes_S"):   # optional: NULL selects the block solve
+            if getattr(ops, name, None) is not None:
+                setattr(s, name, arr(name))
         h = ctypes.c_void_p()
         tun = None
         if tuning:
@@ -86,6 +89,7 @@ class Plan:
             tun.chain_segment = int(tuning.get("chain_segment", 0))
             tun.solve_instances = int(tuning.get("solve_instances", 0))
             tun.sub_batches = int(tuning.get("sub_batches", 0))
+            tun.nab2_solve = int(tuning.get("nab2_solve", 0))
         _lib.check(L.sddc_plan_create_tuned(ctypes.byref(s), self.precision, int(self.symmetric),
                                             int(max_batch), ctypes.byref(tun) if tun else None,
                                             ctypes.byref(h)))
@@ -100,11 +104,12 @@ class Plan:
 
     def tuning(self):
         """Launch shapes in use: {'chain_segment': rows, 'solve_instances': n
-        (0 = by batch), 'sub_batches': parts of a batch on forked streams in graphs}."""
+        (0 = by batch), 'sub_batches': parts of a batch on forked streams in graphs,
+        'nab2_solve': 1 per-mode block inverses, 2 eigenbasis}."""
         t = _lib.SddcTuning()
         _lib.check(_lib.lib().sddc_plan_tuning(self._h, ctypes.byref(t)))
         return {"chain_segment": t.chain_segment, "solve_instances": t.solve_instances,
-                "sub_batches": t.sub_batches}
+                "sub_batches": t.sub_batches, "nab2_solve": t.nab2_solve}
 
     def close(self):
         """Free the plan (refused, ValueError, while a graph holds its workspace)."""

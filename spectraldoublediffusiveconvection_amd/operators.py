@@ -99,6 +99,20 @@ def nab2_inverses(dt, N_fm, nr, D, R):
     return out
 
 
+def nab2_modes(dt, nr, D, R):
+    """Eigenbasis of P = r^2 - dt r^2 Lap, the mode-independent part of the
+    NAB2 matrices: the block of mode j is P + dt j(j+1) I
+    (Matrix_Operators.py:1014-1030), so Linv_j = V diag(1/(lam + dt j(j+1))) V^-1.
+    Returns one array: lam (nr), then V and V^-1 (nr x nr each, row-major), or
+    None when P has complex eigenvalues (the solve then uses the blocks)."""
+    P = np.diag(R[1:-1] ** 2) - dt * nabla2(D, R)
+    lam, V = np.linalg.eig(P)
+    if np.iscomplexobj(lam) and np.abs(lam.imag).max() > 1e-12 * np.abs(lam).max():
+        return None
+    lam, V = lam.real, np.real(V)
+    return np.ascontiguousarray(np.concatenate([lam, V.ravel(), np.linalg.inv(V).ravel()]))
+
+
 @dataclass
 class OperatorSet:
     N_fm: int
@@ -123,6 +137,8 @@ class OperatorSet:
     Linv_A4: np.ndarray
     Linv_T: np.ndarray
     Linv_S: np.ndarray
+    modes_T: np.ndarray = None     # nab2_modes(dt): the T solve in its eigenbasis
+    modes_S: np.ndarray = None     # nab2_modes(Tau*dt)
 
     @property
     def nr(self):
@@ -167,6 +183,8 @@ def build(N_fm, N_r, d, dt, Pr, Tau):
         Linv_A4=a4_inverses(Pr * dt, N_fm, nr, D, R),
         Linv_T=nab2_inverses(dt, N_fm, nr, D, R),
         Linv_S=nab2_inverses(Tau * dt, N_fm, nr, D, R),
+        modes_T=nab2_modes(dt, nr, D, R),
+        modes_S=nab2_modes(Tau * dt, nr, D, R),
         **pieces,
     )
 

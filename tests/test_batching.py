@@ -58,7 +58,7 @@ def test_sweep_graph_matches_eager_and_alone():
             y = p.step(y, inst["Ra"], inst["Ra_s"], nsteps=5)
         torch.cuda.synchronize()
         assert np.array_equal(y.cpu().numpy()[0], x)
-        assert p.tuning() == tuning
+        assert {k: p.tuning()[k] for k in tuning} == tuning
 
 
 @pytest.mark.gpu
@@ -141,7 +141,7 @@ def test_c4_sweep_256_instances_twelve_classes():
         c = sw.classes[key]
         shape = c.plan.tuning()
         tuned = table.lookup(key, c.B, "mixed")
-        assert tuned is not None and shape == tuned, (key, shape, tuned)   # autotuned shape in use
+        assert tuned is not None and {k: shape[k] for k in tuned} == tuned, (key, shape, tuned)
         o = O.build_operators(*key)
         for idx in idxs[:3]:
             inst = insts[idx]

@@ -4,6 +4,9 @@ size-independent properties at the full workload size.
 
 Tolerances (relative 2-norm unless stated):
   * transforms, theta operators, slabs, solves: 1e-14
+  * T and S solves in the eigenbasis of r^2 - h r^2 Lap (nab2_solve 2):
+    5e-14 (the eigenvectors' condition number is about 2; measured up to
+    1.1e-14 at N_r = 65)
   * nonlinear terms F(X,X) and F(X,dv)+F(dv,X):   1e-13
   * one step:                                     1e-12 (the per-step bound)
   * kinetic energy / Nusselt over a trajectory:   1e-9
@@ -31,6 +34,7 @@ C1 = dict(N_r=20, N_fm=64, d=0.35, Pr=1.0, Tau=1.0 / 15.0, Ra=9000.0, Ra_s=500.0
 # exp(~20 t) and every resolution overflows within ~500 steps (the reference
 # algorithm included); Ra_T = 3000 stays bounded for 10^4 steps.
 RA_LONG = 3000.0
+TOL_NAB2 = 5e-14
 
 
 def rel(a, b):
@@ -151,11 +155,12 @@ def test_solve_segmentations(K, Nr, max_batch, sym):
 @pytest.mark.parametrize("K,Nr", [(64, 20), (512, 40), (10, 9), (32, 65)])
 @pytest.mark.parametrize("segment", [0, 8])
 @pytest.mark.parametrize("sym", [False, True])
-@pytest.mark.parametrize("width", [16, 3, 5, 6])
+@pytest.mark.parametrize("width", [16, 3, 5, 6, 9, 10])
 def test_solve_mfma(K, Nr, segment, sym, width):
     """The 16-instance MFMA chain solve (solve_instances = 16) and the
-    mixed-width quad solve (3: 2 instances per A4 workgroup, 4 per NAB2
-    workgroup) against the oracle's back substitution, for a ragged batch of
+    mixed-width quad solves (3, 5, 6: 2 or 3 instances per A4 workgroup, 4
+    per NAB2 workgroup; 9, 10: one per A4 workgroup with deeper prefetch)
+    against the oracle's back substitution, for a ragged batch of
     21 (partial workgroups), unsegmented and in segments of 8 rows, then
     three steps against the oracle."""
     d, dt, Pr, Tau = 0.35, 5e-3, 1.0, 1 / 15.0
@@ -186,7 +191,58 @@ def test_solve_mfma(K, Nr, segment, sym, width):
         assert rel(Y[b], ref) < 1e-12, b
 
 
-@pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 16])
+@pytest.mark.parametrize("K,Nr", [(64, 20), (512, 40), (10, 9), (32, 65)])
+@pytest.mark.parametrize("sym", [False, True])
+def test_nab2_eigenbasis_solve(K, Nr, sym):
+    """The T and S solves in the eigenbasis (nab2_solve 2) against the
+    per-mode block chains (the default) and the oracle's NAB2_BSub_TSTEP_V2, for a ragged
+    batch of 5 over chunked chains (K = 512: four chunks of 64 rows per
+    chain; N_r = 65: the largest radial size), then three steps of both
+    plans against each other and the oracle."""
+    d, dt, Pr, Tau = 0.35, 5e-3, 1.0, 1 / 15.0
+    o = O.build_operators(K, Nr, d, dt, Pr, Tau)
+    op = ops.build(K, Nr, d, dt, Pr, Tau)
+    B = 5
+    pm = dev.Plan(op, symmetric=sym, max_batch=B, tuning={"nab2_solve": 2})
+    pb = dev.Plan(op, symmetric=sym, max_batch=B)
+    assert pm.tuning()["nab2_solve"] == 2 and pb.tuning()["nab2_solve"] == 1
+    nr = Nr - 1
+    rng = np.random.default_rng(K + Nr)
+    g = rng.standard_normal((B, K * nr)) * np.logspace(0, -3, K).repeat(nr)[None, :]
+    for field, Linv, h in ((1, o.Linv_T, dt), (2, o.Linv_S, Tau * dt)):
+        fm, fb = H(pm.bsub(T(g), field)), H(pb.bsub(T(g), field))
+        for b in range(B):
+            ref = O.nab2_bsub(g[b], Linv, K, nr, h, sym)
+            assert rel(fm[b], ref) < TOL_NAB2, (field, b)
+            assert rel(fb[b], ref) < 1e-14, (field, b)
+    X = 1e-3 * rng.standard_normal((B, 3 * K * nr))
+    Ym = H(pm.step(T(X), RA_LONG, 500.0, nsteps=3))
+    Yb = H(pb.step(T(X), RA_LONG, 500.0, nsteps=3))
+    for b in range(B):
+        ref = X[b]
+        for _ in range(3):
+            ref = O.step(ref, o, RA_LONG, 500.0, symmetric=sym)
+        assert rel(Ym[b], ref) < 1e-12, b
+        assert rel(Ym[b], Yb[b]) < 1e-13, b
+    pm.close()
+    pb.close()
+
+
+def test_nab2_eigenbasis_needs_the_modes():
+    """An operator set without modes_T/modes_S (the drop-ins' caller-supplied
+    matrices) solves with the blocks; asking for the eigenbasis is refused."""
+    o = ops.build(16, 12, 0.35, 5e-3, 1.0, 1.0)
+    bare = ops.from_matrices(o.D, o.R, 16, dt=5e-3, Linv_A4=o.Linv_A4, Linv_T=o.Linv_T,
+                             Linv_S=o.Linv_S)
+    assert bare.modes_T is None
+    assert dev.Plan(bare).tuning()["nab2_solve"] == 1
+    with pytest.raises(ValueError, match="nab2_solve 2"):
+        dev.Plan(bare, tuning={"nab2_solve": 2})
+    with pytest.raises(ValueError, match="nab2_solve must be"):
+        dev.Plan(o, tuning={"nab2_solve": 3})
+
+
+@pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 16])
 def test_solve_widths_padded_instances(width):
     """Regression test: batches that do not fill the last workgroup of a
     width (B = 4 and 5; with three instances per A4 workgroup, instances 4

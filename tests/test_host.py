@@ -93,3 +93,50 @@ def test_output_tensor_checks():
         with pytest.raises(ValueError):
             dev.check_out(out, X)
             pytest.fail(name)
+
+
+def _modal_nab2(g, modes, K, nr, h, symmetric=False):
+    """The NAB2 chain solved as nr scalar recurrences in the eigenbasis of
+    P = r^2 - h r^2 Lap (the form of the device kernel k_nab2_modal)."""
+    lam = modes[:nr]
+    V = modes[nr:nr + nr * nr].reshape(nr, nr)
+    Vi = modes[nr + nr * nr:].reshape(nr, nr)
+    gt = g.reshape(K, nr) @ Vi.T
+    ft = np.zeros_like(gt)
+    for chain in ((1,) if symmetric else (0, 1)):
+        bt = np.zeros(nr)
+        for j in range(K - 1 - chain, -1, -2):
+            q = 1.0 / (lam + h * j * (j + 1))
+            ft[j] = (gt[j] - (0.5 if j == 0 else 1.0) * bt) * q
+            bt = (lam + h * j * (j - 1)) * q * bt + 2.0 * h * j * q * gt[j]
+    return (ft @ V.T).ravel()
+
+
+@pytest.mark.parametrize("K,Nr,d,dt,Tau", [(64, 20, 0.35, 5e-3, 1 / 15.0), (32, 16, 2.0, 1.0, 1.0),
+                                           (48, 30, 0.353, 1.0, 1.0), (512, 40, 0.35, 5e-3, 1 / 15.0)])
+def test_nab2_eigenbasis_reproduces_the_reference_blocks(K, Nr, d, dt, Tau):
+    """operators.nab2_modes against the reference's per-mode inverses
+    (NAB2_TSTEP_MATS, built by the oracle): V diag(1/(lam + h j(j+1))) V^-1
+    equals Linv of every mode, and the scalar-recurrence chain equals the
+    oracle's NAB2_BSub_TSTEP_V2 for both fields, with and without the
+    symmetry mask.  A wrong shift, sign or index of the recurrence fails the
+    second check by orders of magnitude more than the tolerance."""
+    o = O.build_operators(K, Nr, d, dt, 1.0, Tau)
+    ops = operators.build(K, Nr, d, dt, 1.0, Tau)
+    nr = Nr - 1
+    rng = np.random.default_rng(K + Nr)
+    for modes, Linv, h in ((ops.modes_T, o.Linv_T, dt), (ops.modes_S, o.Linv_S, Tau * dt)):
+        assert modes is not None and modes.shape == (nr + 2 * nr * nr,)
+        lam = modes[:nr]
+        V = modes[nr:nr + nr * nr].reshape(nr, nr)
+        Vi = modes[nr + nr * nr:].reshape(nr, nr)
+        assert np.all(lam > 0)
+        for jj in (0, 1, K // 2, K - 2, K - 1):
+            j = K - 1 - jj
+            Lj = (V * (1.0 / (lam + h * j * (j + 1)))) @ Vi
+            assert np.linalg.norm(Lj - Linv[jj]) / np.linalg.norm(Linv[jj]) < 1e-13, jj
+        for sym in (False, True):
+            g = rng.standard_normal(K * nr)
+            ref = O.nab2_bsub(g, Linv, K, nr, h, sym)
+            got = _modal_nab2(g, modes, K, nr, h, sym)
+            assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 5e-14, sym
