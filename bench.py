@@ -63,8 +63,8 @@ def measured_traffic(kernel, B):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
-            rec = json.load(f).get(kernel)
-    except (OSError, ValueError):
+            rec = json.load(f).get(kernel, {}).get(str(B))
+    except (OSError, ValueError, AttributeError):
         return None
     if not rec or rec.get("workload") != "N_r=40 N_fm=512 B=%d mixed" % B:
         return None
