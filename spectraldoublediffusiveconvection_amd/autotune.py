@@ -40,10 +40,10 @@ def bucket(B):
 
 def candidates(N_fm, B=1):
     """Launch shapes tried for a class with N_fm modes (N_fm/2 rows per chain)
-    at batch B: quad-lane solves with 1, 2 or 4 instances per workgroup or the
+    at batch B: quad-lane solves with 1, 2 or 4 instances per workgroup, the
     mixed widths 3, 5 and 6 (2 per A4, 4 per NAB2 workgroup; 5 with the
-    two-exchange A4 chains, 6 with three of them per workgroup), from B = 16
-    on the
+    two-exchange A4 chains, 6 with three of them per workgroup) and 9, 10
+    (one instance per A4 workgroup with deeper prefetch), from B = 16 on the
     16-instance MFMA solve, and from B = 64 on batches split in 2 or 4 parts
     on forked streams of the graph."""
     n = N_fm // 2
@@ -51,10 +51,33 @@ def candidates(N_fm, B=1):
     if B < 64:   # short segments only pay off while the device is not full
         segs.add(max(8, 4 * math.ceil(math.sqrt(n) / 4)))
     segs = sorted(segs)
-    nbs = (1, 2, 3, 4, 5, 6, 16) if B >= 16 else (1, 2, 3, 4, 5, 6)
+    nbs = (1, 2, 3, 4, 5, 6, 9, 10, 16) if B >= 16 else (1, 2, 3, 4, 5, 6, 9, 10)
     subs = (1, 2, 4) if B >= 64 else (1,)
     return [{"chain_segment": m, "solve_instances": nb, "sub_batches": sb}
             for m in segs for nb in nbs for sb in subs]
+
+
+def refine_class(table, N_fm, N_r, B, widths, precision="mixed", steps=GRAPH_STEPS, margin=0.02):
+    """Time the table's shape for one class and batch again beside the given
+    solve widths at the candidate segment lengths (same sub-batches), and
+    keep the fastest; a new shape must win by `margin` (timing noise of a
+    40-step graph is about 1%).  Returns (tuning, {shape: µs})."""
+    cur = table.lookup((N_fm, N_r), B, precision)
+    if cur is None:
+        return tune_class(N_fm, N_r, B, precision, steps=steps)
+    e = table.entries[_key(N_fm, N_r, precision, B)]
+    timings = dict(e.get("timings_us", {}))
+    best_t = time_shape(N_fm, N_r, B, precision, cur, steps=steps)
+    best = cur
+    segs = sorted({c["chain_segment"] for c in candidates(N_fm, B)})
+    for m in segs:
+        for nb in widths:
+            t = {"chain_segment": m, "solve_instances": nb, "sub_batches": cur.get("sub_batches", 1)}
+            us = time_shape(N_fm, N_r, B, precision, t, steps=steps)
+            timings["seg%d_nb%d_sb%d" % (m, nb, t["sub_batches"])] = round(us, 2)
+            if us < best_t * (1.0 - margin):
+                best, best_t = t, us
+    return best, timings
 
 
 def _key(N_fm, N_r, precision, B):
@@ -139,12 +162,18 @@ def main():
     ap.add_argument("--N_fm", type=int, nargs="*", default=[64, 128, 256, 512])
     ap.add_argument("--batches", type=int, nargs="*", default=list(BUCKETS))
     ap.add_argument("--steps", type=int, default=GRAPH_STEPS, help="steps per timed graph")
+    ap.add_argument("--refine", type=int, nargs="*", default=None,
+                    help="only compare the table's shapes with these solve widths")
     args = ap.parse_args()
     table = TuningTable(args.out)
     for N_r in args.N_r:
         for N_fm in args.N_fm:
             for B in args.batches:
-                t, tm = tune_class(N_fm, N_r, B, args.precision, steps=args.steps)
+                if args.refine:
+                    t, tm = refine_class(table, N_fm, N_r, B, args.refine, args.precision,
+                                         steps=args.steps)
+                else:
+                    t, tm = tune_class(N_fm, N_r, B, args.precision, steps=args.steps)
                 table.put(N_fm, N_r, args.precision, B, t, tm)
                 print("N_r=%d N_fm=%d B=%d -> %s  %s" % (N_r, N_fm, B, t, tm), flush=True)
                 table.save()
