@@ -14,8 +14,9 @@ def test_autotune_buckets_and_candidates(tmp_path):
     assert [autotune.bucket(b) for b in (1, 2, 4, 5, 16, 17, 300)] == [1, 4, 4, 16, 16, 64, 256]
     c = autotune.candidates(512)
     segs = sorted({t["chain_segment"] for t in c})
-    assert segs == [16, 32, 256] and {t["solve_instances"] for t in c} == {1, 2, 3, 4, 5, 6}
-    assert {t["solve_instances"] for t in autotune.candidates(512, 64)} == {1, 2, 3, 4, 5, 6, 16}
+    assert segs == [16, 32, 256] and {t["solve_instances"] for t in c} == {1, 2, 3, 4, 5, 6, 9, 10}
+    assert {t["solve_instances"] for t in autotune.candidates(512, 64)} == {1, 2, 3, 4, 5, 6, 9, 10,
+                                                                            16}
     t = autotune.TuningTable(str(tmp_path / "t.json"))
     t.put(64, 20, "mixed", 10, {"chain_segment": 8, "solve_instances": 2}, {"x": 1.0})
     t.save()
