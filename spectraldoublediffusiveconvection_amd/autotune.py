@@ -65,9 +65,12 @@ def refine_class(table, N_fm, N_r, B, widths, precision="mixed", steps=GRAPH_STE
     cur = table.lookup((N_fm, N_r), B, precision)
     if cur is None:
         return tune_class(N_fm, N_r, B, precision, steps=steps)
-    e = table.entries[_key(N_fm, N_r, precision, B)]
-    timings = dict(e.get("timings_us", {}))
+    # every timing of the entry comes from this run: the table's shape is
+    # timed again, and shapes not timed again are dropped
+    timings = {}
     best_t = time_shape(N_fm, N_r, B, precision, cur, steps=steps)
+    timings["seg%d_nb%d_sb%d" % (cur["chain_segment"], cur["solve_instances"],
+                                 cur.get("sub_batches", 1))] = round(best_t, 2)
     best = cur
     segs = sorted({c["chain_segment"] for c in candidates(N_fm, B)})
     for m in segs:

@@ -209,9 +209,14 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
       }
       lds_barrier();
       const double alpha = 2.0 * dtf * j;
+      double fr[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) fr[t] = pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) fr[t] = quad_sum(fr[t]);
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]));
+        const double f = fr[t];
         double* dst = (live && rv) ? in.x[t] + (size_t)j * nr + r : sink;
         *dst = f;
         b[t] = live ? b[t] + alpha * f : b[t];
@@ -287,10 +292,21 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
         xs[k & 1][t][wp] = gv[d][t] + bb;
       }
       lds_barrier();
+      double fr[NB], dr[NB];
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]));
-        const double df = quad_sum(pdot<S::NQ>(mblk[d], &xs[k & 1][t][kq * S::CPL]));
+        fr[t] = pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]);
+        dr[t] = pdot<S::NQ>(mblk[d], &xs[k & 1][t][kq * S::CPL]);
+      }
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        fr[t] = quad_sum(fr[t]);
+        dr[t] = quad_sum(dr[t]);
+      }
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const double f = fr[t];
+        const double df = dr[t];
         double* dst = (live && rv) ? in.x[t] + (size_t)row * nr + r : sink;
         *dst = f;
         const double nbfe = bfe[t] + bj * f + bjt * fe[t];
@@ -374,17 +390,28 @@ __device__ __forceinline__ void local_a4x(const ChainArgs& a, int start, int k0,
         xs[0][t][wp] = fe[t];
       }
       lds_barrier();
+      // the instances' products in separate loops, so that their dependent
+      // chains interleave instead of running one after the other
+      double d2fe[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) d2fe[t] = pdot<S::NQ>(d2, &xs[0][t][kq * S::CPL]);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) d2fe[t] = quad_sum(d2fe[t]);
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const double d2fe = quad_sum(pdot<S::NQ>(d2, &xs[0][t][kq * S::CPL]));
-        const double l1fe = d2fe + bj * (ir4 * fe[t]);
+        const double l1fe = d2fe[t] + bj * (ir4 * fe[t]);
         const double bb = (dta * bjt) * (l1fe + ir4 * bfe[t]) - bjt * (ir2 * fe[t]);
         xs[1][t][wp] = gv[d][t] + bb;
       }
       lds_barrier();
+      double fr[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) fr[t] = pdot<S::NQ>(blk[d], &xs[1][t][kq * S::CPL]);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) fr[t] = quad_sum(fr[t]);
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const double f = quad_sum(pdot<S::NQ>(blk[d], &xs[1][t][kq * S::CPL]));
+        const double f = fr[t];
         double* dst = (live && rv) ? in.x[t] + (size_t)row * nr + r : sink;
         *dst = f;
         const double nbfe = bfe[t] + bj * f + bjt * fe[t];

@@ -242,6 +242,28 @@ def test_nab2_eigenbasis_needs_the_modes():
         dev.Plan(o, tuning={"nab2_solve": 3})
 
 
+@pytest.mark.parametrize("group", [(1, 2, 4, 10), (5, 7, 8, 9)])
+def test_solve_width_groups_give_the_same_bits(group):
+    """Widths that differ only in how many instances share a workgroup or in
+    how many rows of operator data are in flight do the same arithmetic per
+    instance: the steps of a ragged batch agree bit for bit within each
+    group (unsegmented and in segments of 8 rows)."""
+    K, Nr, d, dt, Pr, Tau = 64, 20, 0.35, 5e-3, 1.0, 1 / 15.0
+    op = ops.build(K, Nr, d, dt, Pr, Tau)
+    rng = np.random.default_rng(sum(group))
+    B = 7
+    X = 1e-3 * rng.standard_normal((B, 3 * K * (Nr - 1)))
+    Ra = np.linspace(2500.0, 9000.0, B)
+    for seg in (0, 8):
+        outs = []
+        for w in group:
+            p = dev.Plan(op, max_batch=B, tuning={"chain_segment": seg, "solve_instances": w})
+            outs.append(H(p.step(T(X), Ra, 500.0, nsteps=3)))
+            p.close()
+        for w, o in zip(group[1:], outs[1:]):
+            assert np.array_equal(o, outs[0]), (seg, w)
+
+
 @pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 16])
 def test_solve_widths_padded_instances(width):
     """Regression test: batches that do not fill the last workgroup of a
