@@ -93,9 +93,8 @@ __global__ void k_combine(double* __restrict__ y, const double* __restrict__ V,
   }
 }
 
-// z = a*x + b*y
-__global__ void k_axpby(double* __restrict__ z, const double* __restrict__ x,
-                        const double* __restrict__ y, int n, double a, double b) {
+// z = a*x + b*y; z may be x or y (in-place updates), so none is restrict
+__global__ void k_axpby(double* z, const double* x, const double* y, int n, double a, double b) {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < (size_t)n;
        e += (size_t)gridDim.x * blockDim.x)
     z[e] = a * x[e] + (y ? b * y[e] : 0.0);

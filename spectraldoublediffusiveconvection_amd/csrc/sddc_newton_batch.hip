@@ -103,9 +103,9 @@ __global__ void k_bcombine(double* __restrict__ y, const double* __restrict__ V,
   }
 }
 
-// z[b] = a x[b] + b y[b] for gated instances (y may be null)
-__global__ void k_baxpby(double* __restrict__ z, const double* __restrict__ x,
-                         const double* __restrict__ y, int n, double a, double c,
+// z[b] = a x[b] + b y[b] for gated instances (y may be null).  z may be x or
+// y (in-place updates), so the three are not declared restrict.
+__global__ void k_baxpby(double* z, const double* x, const double* y, int n, double a, double c,
                          const int* __restrict__ fl, int gate) {
   const int b = blockIdx.y;
   if (gate >= 0 && !fl[(size_t)b * kNF + gate]) return;
