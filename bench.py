@@ -12,8 +12,12 @@ seeded normal vectors scaled to ||X||_2 = 1e-3, resident in HBM before the
 timed region.  ``value`` is the aggregate instance-steps per second.
 
 Beside it the line reports the plain float64 single-instance step (the
-baseline the metric is quoted against), the roofline of the dominant kernel
-and the CPU baseline (the oracle's NumPy step on one host core, bounded sample).
+baseline the metric is quoted against), the same batch in float64, the
+roofline of the dominant kernel (at the graph's sub-batch shape and on the
+full batch), the mixed-vs-FP64 error in convecting flow, the mixed-size
+sweep, the serving legs and the CPU baseline (the oracle's NumPy step: one
+host core, and single-thread processes on every core of the affinity set,
+bounded samples).
 """
 from __future__ import annotations
 

@@ -161,3 +161,25 @@ def test_c4_sweep_256_instances_twelve_classes():
             alone.close()
     print("C4 sweep: 12 classes, worst relative error after 3 steps vs oracle: mixed %.2e, "
           "float64 %.2e" % (worst_mixed, worst64))
+
+
+def test_autotune_refine_keeps_a_shape_unless_beaten_by_the_margin(tmp_path, monkeypatch):
+    """refine_class times the table's shape again beside the new widths and
+    replaces it only when a new shape is faster by more than the margin; the
+    entry's timings all come from that run (no timing is simulated beyond
+    the stand-in for the GPU timer)."""
+    t = autotune.TuningTable(str(tmp_path / "t.json"))
+    cur = {"chain_segment": 16, "solve_instances": 1, "sub_batches": 1}
+    t.put(512, 40, "mixed", 1, cur, {"stale": 1.0})
+    fake = {1: 100.0, 9: 99.0, 10: 90.0}
+
+    def timer(N_fm, N_r, B, precision, tuning, steps=40):
+        return fake[tuning["solve_instances"]] + (0.5 if tuning["chain_segment"] != 16 else 0.0)
+
+    monkeypatch.setattr(autotune, "time_shape", timer)
+    best, tm = autotune.refine_class(t, 512, 40, 1, (9, 10))
+    assert best == {"chain_segment": 16, "solve_instances": 10, "sub_batches": 1}
+    assert tm["seg16_nb1_sb1"] == 100.0 and "stale" not in tm
+    fake[10] = 99.0          # within 2% of the current shape: kept
+    best, _ = autotune.refine_class(t, 512, 40, 1, (9, 10))
+    assert best == cur
