@@ -111,13 +111,17 @@ typedef struct sddc_plan sddc_plan;
  *                    in the eigenbasis of the operator set's modes_T /
  *                    modes_S (equal to rounding, about 1e-14 relative; a
  *                    separate launch after the A4 chains, about three times
- *                    faster than the block chains on its own). */
+ *                    faster than the block chains on its own).
+ *   spectral_instances  instances per workgroup of the spectral stage (1-4;
+ *                    0: from the number of (mode tile, instance) pairs).
+ *                    The same bits whatever the value. */
 typedef struct sddc_tuning {
   int chain_segment;
   int solve_instances;
   int sub_batches;
   int nab2_solve;
-  int reserved[4];
+  int spectral_instances;
+  int reserved[3];
 } sddc_tuning;
 
 /* Plan lifecycle.  Replaces the operator tuple of Build_Matrix_Operators

@@ -40,7 +40,7 @@ class SddcOperators(ctypes.Structure):
 class SddcTuning(ctypes.Structure):
     _fields_ = [("chain_segment", ctypes.c_int), ("solve_instances", ctypes.c_int),
                 ("sub_batches", ctypes.c_int), ("nab2_solve", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("spectral_instances", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class SddcKrylovOpts(ctypes.Structure):

@@ -57,30 +57,57 @@ def candidates(N_fm, B=1):
             for m in segs for nb in nbs for sb in subs]
 
 
-def refine_class(table, N_fm, N_r, B, widths, precision="mixed", steps=GRAPH_STEPS, margin=0.02):
-    """Time the table's shape for one class and batch again beside the given
-    solve widths at the candidate segment lengths (same sub-batches), and
-    keep the fastest; a new shape must win by `margin` (timing noise of a
-    40-step graph is about 1%).  Returns (tuning, {shape: µs})."""
+def shape_key(t):
+    """Timing key of a launch shape."""
+    k = "seg%d_nb%d_sb%d" % (t["chain_segment"], t["solve_instances"], t.get("sub_batches", 1))
+    if t.get("spectral_instances", 0):
+        k += "_sp%d" % t["spectral_instances"]
+    return k
+
+
+def spectral_per_group(N_fm, B, spectral_instances):
+    """Instances one spectral workgroup steps through at batch B: the
+    library's automatic choice for 0 (from the (mode tile, instance) pairs),
+    and never more than B."""
+    if spectral_instances <= 0:
+        tiles = (N_fm + 15) // 16
+        return max(1, min(4, tiles * B // 512))
+    return min(spectral_instances, B)
+
+
+def refine_class(table, N_fm, N_r, B, widths=(), precision="mixed", steps=GRAPH_STEPS,
+                 margin=0.05, spectral=(), rounds=3):
+    """Time the table's shape for one class and batch again beside variants
+    of it -- the given solve widths at the candidate segment lengths, and the
+    given spectral-stage instances per workgroup -- and keep the fastest; a
+    new shape must win by `margin`.  The shapes are timed in `rounds`
+    interleaved rounds and each keeps its best.  Repeated plans of one shape
+    at batches of 1-16 differ by up to 15% (each plan gets new allocations),
+    hence the 5% margin; spectral variants that run the same launch as the
+    table's shape are not timed.  Every timing of the entry comes from this
+    run.  Returns (tuning, {shape: µs})."""
     cur = table.lookup((N_fm, N_r), B, precision)
     if cur is None:
         return tune_class(N_fm, N_r, B, precision, steps=steps)
-    # every timing of the entry comes from this run: the table's shape is
-    # timed again, and shapes not timed again are dropped
-    timings = {}
-    best_t = time_shape(N_fm, N_r, B, precision, cur, steps=steps)
-    timings["seg%d_nb%d_sb%d" % (cur["chain_segment"], cur["solve_instances"],
-                                 cur.get("sub_batches", 1))] = round(best_t, 2)
-    best = cur
-    segs = sorted({c["chain_segment"] for c in candidates(N_fm, B)})
-    for m in segs:
+    shapes = {shape_key(cur): cur}
+    for m in sorted({c["chain_segment"] for c in candidates(N_fm, B)}):
         for nb in widths:
-            t = {"chain_segment": m, "solve_instances": nb, "sub_batches": cur.get("sub_batches", 1)}
-            us = time_shape(N_fm, N_r, B, precision, t, steps=steps)
-            timings["seg%d_nb%d_sb%d" % (m, nb, t["sub_batches"])] = round(us, 2)
-            if us < best_t * (1.0 - margin):
-                best, best_t = t, us
-    return best, timings
+            t = dict(cur, chain_segment=m, solve_instances=nb)
+            shapes.setdefault(shape_key(t), t)
+    eff = spectral_per_group(N_fm, B, cur.get("spectral_instances", 0))
+    for sp in spectral:
+        if spectral_per_group(N_fm, B, sp) == eff:
+            continue
+        t = dict(cur, spectral_instances=sp)
+        shapes.setdefault(shape_key(t), t)
+    times = {k: float("inf") for k in shapes}
+    for _ in range(max(1, rounds)):
+        for k, t in shapes.items():
+            times[k] = min(times[k], time_shape(N_fm, N_r, B, precision, t, steps=steps))
+    ck = shape_key(cur)
+    wins = [k for k in shapes if times[k] < times[ck] * (1.0 - margin)]
+    best_k = min(wins, key=times.get) if wins else ck
+    return shapes[best_k], {k: round(v, 2) for k, v in times.items()}
 
 
 def _key(N_fm, N_r, precision, B):
@@ -121,8 +148,11 @@ def _ops_for(N_fm, N_r, d, dt):
     return _OPS[key]
 
 
-def time_shape(N_fm, N_r, B, precision, tuning, steps=GRAPH_STEPS, reps=3, d=0.35, dt=5e-3):
-    """Best µs per batched step under one launch shape (graph replay)."""
+def time_shape(N_fm, N_r, B, precision, tuning, steps=GRAPH_STEPS, reps=3, d=0.35, dt=5e-3,
+               min_seconds=0.03):
+    """Best µs per batched step under one launch shape (graph replay): the
+    best of `reps` launches, and of as many more as fill `min_seconds`, so
+    that the short graphs of small batches are timed over enough launches."""
     import torch
 
     from . import device as dev
@@ -133,12 +163,14 @@ def time_shape(N_fm, N_r, B, precision, tuning, steps=GRAPH_STEPS, reps=3, d=0.3
     g = plan.graph(X, np.linspace(2500.0, 3000.0, B), 500.0, nsteps=steps)
     g.launch()
     torch.cuda.synchronize()
-    best = float("inf")
-    for _ in range(reps):
+    best, n, spent = float("inf"), 0, 0.0
+    while n < reps or spent < min_seconds:
         t0 = time.perf_counter()
         g.launch()
         torch.cuda.synchronize()
-        best = min(best, (time.perf_counter() - t0) / steps * 1e6)
+        el = time.perf_counter() - t0
+        best = min(best, el / steps * 1e6)
+        n, spent = n + 1, spent + el
     g.close()
     plan.close()
     return best
@@ -150,8 +182,7 @@ def tune_class(N_fm, N_r, B, precision="mixed", steps=GRAPH_STEPS):
     best, best_t = None, float("inf")
     for t in candidates(N_fm, B):
         us = time_shape(N_fm, N_r, B, precision, t, steps=steps)
-        timings["seg%d_nb%d_sb%d" % (t["chain_segment"], t["solve_instances"],
-                                     t["sub_batches"])] = round(us, 2)
+        timings[shape_key(t)] = round(us, 2)
         if us < best_t:
             best, best_t = t, us
     return best, timings
@@ -167,14 +198,16 @@ def main():
     ap.add_argument("--steps", type=int, default=GRAPH_STEPS, help="steps per timed graph")
     ap.add_argument("--refine", type=int, nargs="*", default=None,
                     help="only compare the table's shapes with these solve widths")
+    ap.add_argument("--refine-spectral", type=int, nargs="*", default=None,
+                    help="only compare the table's shapes with these spectral instances")
     args = ap.parse_args()
     table = TuningTable(args.out)
     for N_r in args.N_r:
         for N_fm in args.N_fm:
             for B in args.batches:
-                if args.refine:
-                    t, tm = refine_class(table, N_fm, N_r, B, args.refine, args.precision,
-                                         steps=args.steps)
+                if args.refine is not None or args.refine_spectral is not None:
+                    t, tm = refine_class(table, N_fm, N_r, B, args.refine or (), args.precision,
+                                         steps=args.steps, spectral=args.refine_spectral or ())
                 else:
                     t, tm = tune_class(N_fm, N_r, B, args.precision, steps=args.steps)
                 table.put(N_fm, N_r, args.precision, B, t, tm)

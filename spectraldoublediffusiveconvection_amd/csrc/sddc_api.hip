@@ -225,6 +225,7 @@ int sddc_plan_tuning(const sddc_plan* plan, sddc_tuning* out) {
   out->solve_instances = plan->p.tune_solve_nbi;
   out->sub_batches = plan->p.tune_sub_batches;
   out->nab2_solve = plan->p.nab2_modal ? 2 : 1;
+  out->spectral_instances = plan->p.tune_spectral_nb;
   return SDDC_OK;
 }
 
@@ -310,6 +311,9 @@ int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetr
     return bail(fail(SDDC_ERR_INVALID, "sub_batches must be in [0, 8]"));
   if (tuning && (tuning->nab2_solve < 0 || tuning->nab2_solve > 2))
     return bail(fail(SDDC_ERR_INVALID, "nab2_solve must be 0, 1 or 2"));
+  if (tuning && (tuning->spectral_instances < 0 || tuning->spectral_instances > 4))
+    return bail(fail(SDDC_ERR_INVALID, "spectral_instances must be in [0, 4]"));
+  p.tune_spectral_nb = tuning ? tuning->spectral_instances : 0;
   {
     const bool have_modes = ops->modes_T && ops->modes_S;
     const int want = tuning ? tuning->nab2_solve : 0;

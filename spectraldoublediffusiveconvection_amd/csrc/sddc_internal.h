@@ -138,6 +138,7 @@ struct Plan {
   int ex_batch = 0;                  // instances chain_EX holds
   int tune_solve_nbi = 0;            // instances per solve workgroup, 0 = automatic
   int tune_sub_batches = 1;          // parts of a batch on forked streams in graphs
+  int tune_spectral_nb = 0;          // instances per spectral workgroup, 0 = automatic
   size_t ws_bytes = 0;
 };
 

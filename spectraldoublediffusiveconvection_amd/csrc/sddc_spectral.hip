@@ -416,7 +416,7 @@ template <typename R>
 static int launch_spectral_r(const Plan& p, SpectralArgs a, int B, hipStream_t s) {
   const int tiles = (p.K + kTile - 1) / kTile;
   a.B = B;
-  a.nb = spectral_instances(tiles, B);
+  a.nb = p.tune_spectral_nb > 0 ? p.tune_spectral_nb : spectral_instances(tiles, B);
   const dim3 grid(tiles, (B + a.nb - 1) / a.nb);
   switch (p.NRP) {
     case 16: hipLaunchKernelGGL((k_spectral<R, 16>), grid, dim3(320), 0, s, a); break;

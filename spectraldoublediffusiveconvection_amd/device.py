@@ -90,6 +90,7 @@ class Plan:
             tun.solve_instances = int(tuning.get("solve_instances", 0))
             tun.sub_batches = int(tuning.get("sub_batches", 0))
             tun.nab2_solve = int(tuning.get("nab2_solve", 0))
+            tun.spectral_instances = int(tuning.get("spectral_instances", 0))
         _lib.check(L.sddc_plan_create_tuned(ctypes.byref(s), self.precision, int(self.symmetric),
                                             int(max_batch), ctypes.byref(tun) if tun else None,
                                             ctypes.byref(h)))
@@ -105,11 +106,13 @@ class Plan:
     def tuning(self):
         """Launch shapes in use: {'chain_segment': rows, 'solve_instances': n
         (0 = by batch), 'sub_batches': parts of a batch on forked streams in graphs,
-        'nab2_solve': 1 per-mode block inverses, 2 eigenbasis}."""
+        'nab2_solve': 1 per-mode block inverses, 2 eigenbasis,
+        'spectral_instances': instances per spectral workgroup (0 = by batch)}."""
         t = _lib.SddcTuning()
         _lib.check(_lib.lib().sddc_plan_tuning(self._h, ctypes.byref(t)))
         return {"chain_segment": t.chain_segment, "solve_instances": t.solve_instances,
-                "sub_batches": t.sub_batches, "nab2_solve": t.nab2_solve}
+                "sub_batches": t.sub_batches, "nab2_solve": t.nab2_solve,
+                "spectral_instances": t.spectral_instances}
 
     def close(self):
         """Free the plan (refused, ValueError, while a graph holds its workspace)."""

@@ -107,8 +107,8 @@ def class_bytes(N_fm, N_r, capacity, precision):
 
 
 def slot_buckets(capacity):
-    """Slot counts with a captured graph: 1, 4, 16, then multiples of 16 up
-    to the capacity (which is always one)."""
+    """Slot counts with a captured graph: 1, 4, 16, 32, 64, then steps of
+    64, below the capacity, and the capacity itself."""
     out = [b for b in (1, 4) if b < capacity]
     b = 16
     while b < capacity:

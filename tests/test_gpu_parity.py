@@ -264,6 +264,25 @@ def test_solve_width_groups_give_the_same_bits(group):
             assert np.array_equal(o, outs[0]), (seg, w)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_spectral_instances_give_the_same_bits(precision):
+    """Instances per spectral workgroup (1-4, and the automatic choice) only
+    change which workgroup computes an instance: a ragged batch of 7 steps
+    to the same bits with each."""
+    K, Nr, d, dt, Pr, Tau = 128, 20, 0.35, 5e-3, 1.0, 1 / 15.0
+    op = ops.build(K, Nr, d, dt, Pr, Tau)
+    B = 7
+    X = 1e-3 * np.random.default_rng(5).standard_normal((B, 3 * K * (Nr - 1)))
+    outs = []
+    for sp in (0, 1, 2, 3, 4):
+        p = dev.Plan(op, precision=precision, max_batch=B, tuning={"spectral_instances": sp})
+        assert p.tuning()["spectral_instances"] == sp
+        outs.append(H(p.step(T(X), np.linspace(2500.0, 3000.0, B), 500.0, nsteps=3)))
+        p.close()
+    for sp, o in zip((1, 2, 3, 4), outs[1:]):
+        assert np.array_equal(o, outs[0]), sp
+
+
 @pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 16])
 def test_solve_widths_padded_instances(width):
     """Regression test: batches that do not fill the last workgroup of a
