@@ -739,6 +739,9 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
   for (int c = 0; c < nplans; ++c) {
     if (!plans[c]) return fail(SDDC_ERR_INVALID, "null plan");
     if (int rc = check_batch(plans[c]->p, B[c])) return rc;
+    // the eigenbasis T/S solve has not been validated under graph capture
+    if (plans[c]->p.nab2_modal)
+      return fail(SDDC_ERR_UNSUPPORTED, "graphs of plans with nab2_solve 2 are not supported");
   }
   // no allocation, free or device synchronisation of this library may run
   // on another thread between the capture's begin and end

@@ -228,6 +228,47 @@ def test_nab2_eigenbasis_solve(K, Nr, sym):
     pb.close()
 
 
+@pytest.mark.skip(reason="capturing a graph of an eigenbasis plan, with that solve forked onto "
+                         "a side stream per sub-batch, ended the process with a segmentation fault "
+                         "inside sddc_graph_create on MI355X; the cause was not found, the fork "
+                         "was removed and graph capture now refuses such plans")
+def test_nab2_eigenbasis_beside_the_a4_chains_in_graph_parts():
+    """The eigenbasis solve runs on a side stream beside the A4 chains: a
+    graph of two sub-batches (each with its own side stream) replays to the
+    bits of the eager steps, twice, and both agree with the block form to
+    rounding."""
+    K, Nr, d, dt, Pr, Tau = 64, 20, 0.35, 5e-3, 1.0, 1 / 15.0
+    op = ops.build(K, Nr, d, dt, Pr, Tau)
+    B = 32
+    pm = dev.Plan(op, max_batch=B, tuning={"nab2_solve": 2, "sub_batches": 2})
+    pb = dev.Plan(op, max_batch=B)
+    X = 1e-3 * np.random.default_rng(3).standard_normal((B, 3 * K * (Nr - 1)))
+    Ra = np.linspace(2500.0, 3000.0, B)
+    eager = T(X)
+    xg = T(X).clone()
+    gr = pm.graph(xg, Ra, 500.0, nsteps=3)
+    for rep in range(2):
+        gr.launch()
+        eager = pm.step(eager, Ra, 500.0, nsteps=3)
+        assert np.array_equal(H(xg), H(eager)), rep
+    gr.close()
+    ref = H(pb.step(T(X), Ra, 500.0, nsteps=6))
+    assert rel(H(eager), ref) < 1e-12
+    pm.close()
+    pb.close()
+
+
+def test_graph_capture_refuses_eigenbasis_plans():
+    """Graph capture checks its plans before it captures anything: a plan
+    with nab2_solve 2 is refused with NotImplementedError."""
+    op = ops.build(64, 20, 0.35, 5e-3, 1.0, 1 / 15.0)
+    pm = dev.Plan(op, max_batch=4, tuning={"nab2_solve": 2})
+    X = T(1e-3 * np.random.default_rng(4).standard_normal((4, 3 * 64 * 19)))
+    with pytest.raises(NotImplementedError, match="nab2_solve 2"):
+        pm.graph(X, 2500.0, 500.0, nsteps=2)
+    pm.close()
+
+
 def test_nab2_eigenbasis_needs_the_modes():
     """An operator set without modes_T/modes_S (the drop-ins' caller-supplied
     matrices) solves with the blocks; asking for the eigenbasis is refused."""
