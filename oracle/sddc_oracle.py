@@ -24,6 +24,10 @@ How each part is pinned (tests/test_oracle.py):
   * nonlinear terms, step, kinetic energy, Nusselt number: golden vectors
     written from the reference by tests/golden/make_golden.py, plus
     bilinearity / directional-derivative / symmetry invariants;
+  * linearised step: the central difference of the step, which it equals
+    up to rounding because the nonlinear term is quadratic;
+  * back substitutions: the golden step vectors, ``L @ L_inv = I``, and the
+    eigenbasis form of the NAB2 chain (tests/test_host.py);
   * Newton: residual at convergence (scipy ``lgmres`` is third-party; pinned
     at solution level only);
   * pseudo-arclength step: at solution level, F(X, Ra) = 0, the arclength
