@@ -29,7 +29,14 @@
 // Rows r >= nr are padding: their matrix columns are zero, their lanes read
 // the operator entries and inputs of row nr-1 (cache lines the valid lanes
 // fetch anyway), so their values stay finite, never reach a real row through
-// the zero columns, and are written to a sink without selects or branches.
+// the zero columns, and are not stored: the block chains of the local launch
+// mask those lanes off their stores; the other kernels send them to a sink.
+// The local block chains also run their rows in full groups of prefetched
+// rows with no per-row liveness selects (the last, partial group checks),
+// and address every load and store as a wave-uniform row base plus a fixed
+// 32-bit lane offset.  That form fits the compiler's scalar-base addressing,
+// so a row spends no vector instructions on addresses, and the kernel needs
+// 140 instead of 178 VGPRs (three wavefronts per SIMD instead of two).
 // A matrix-vector product is one partial dot product per lane (NC/4 terms)
 // and a quad reduction (two DPP exchanges, which leave the same sum in all
 // four lanes).  The vector is broadcast through LDS, stored as [c%4][c/4] so
@@ -125,6 +132,25 @@ __device__ __forceinline__ double pdot_reg(const double2 (&a)[NQ], const double 
   return s0 + s1;
 }
 
+// A wave-uniform pointer plus a lane's byte offset: the compiler can then
+// address with a scalar base and one 32-bit vector offset, so a loop that
+// moves the base from row to row spends no vector instructions on addresses.
+__device__ __forceinline__ const double* lane_at(const double* base, unsigned byte_off) {
+  return reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+__device__ __forceinline__ double* lane_at(double* base, unsigned byte_off) {
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(base) + byte_off);
+}
+
+// A loop-invariant lane offset made opaque to loop-invariant code motion:
+// otherwise the compiler adds it to each uniform base once, outside the
+// loop, as a 64-bit vector address that every row then advances with vector
+// adds.  No instruction is emitted.
+__device__ __forceinline__ unsigned opaque(unsigned v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <int NB>
 struct Inst {
   const double* g[NB];
@@ -177,15 +203,17 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
   const bool writer = kq == 0 && r < NRS;
   const int wp = writer ? vpos(r, S::CPL) : NRS + lane;
   const double dtf = a.dtf[field];
-  const double* lb = a.Lq[field] + w * S::NQ * 128 + 2 * lq;
-  double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
+  // wave-uniform row bases, lane parts as byte offsets
+  const double* lb = a.Lq[field] + w * S::NQ * 128;
+  const unsigned lo = 16u * lq, go = 8u * rr, so = 8u * r;
   double2 blk[KD][S::NQ];
   double gv[KD][NB];
   auto fetch = [&](int k, double2 (&bq)[S::NQ], double (&g)[NB]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
-    load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
+    load_blk<S::NQ>(bq, lane_at(lb + (size_t)(K - 1 - j) * S::BLK, opaque(lo)));
+    const unsigned gq = opaque(go);
 #pragma unroll
-    for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
+    for (int t = 0; t < NB; ++t) g[t] = *lane_at(in.g[t] + (size_t)j * nr, gq);
   };
 #pragma unroll
   for (int d = 0; d < KD; ++d) fetch(k0 + d, blk[d], gv[d]);
@@ -193,41 +221,45 @@ __device__ __forceinline__ void local_nab2(const ChainArgs& a, int field, int st
   double b[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) b[t] = 0.0;
-  for (int kb = k0; kb < k1; kb += KD) {
+  // one row of the chain; live is false only for the rows of the last group
+  // that lie past the segment's end (uniform over the workgroup)
+  auto row = [&](int k, int d, bool live) {
+    const int j = start - 2 * (live ? k : k1 - 1);
+    // NAB2 (Matrix_Operators.py:1061-1083): f_j = Linv (g_j - c_j b),
+    // c_0 = 1/2; then b += 2 dt j f_j for row j-2.
+    const double cj = (j == 0) ? 0.5 : 1.0;
 #pragma unroll
-    for (int d = 0; d < KD; ++d) {
-      const int k = kb + d;
-      const bool live = k < k1;
-      const int j = start - 2 * (live ? k : k1 - 1);
-      // NAB2 (Matrix_Operators.py:1061-1083): f_j = Linv (g_j - c_j b),
-      // c_0 = 1/2; then b += 2 dt j f_j for row j-2.
-      const double cj = (j == 0) ? 0.5 : 1.0;
+    for (int t = 0; t < NB; ++t) xs[k & 1][t][wp] = gv[d][t] - cj * b[t];
+    lds_barrier();
+    const double alpha = 2.0 * dtf * j;
+    double fr[NB];
 #pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        const double rhs = gv[d][t] - cj * b[t];
-        xs[k & 1][t][wp] = rhs;
+    for (int t = 0; t < NB; ++t) fr[t] = pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) fr[t] = quad_sum(fr[t]);
+    if (live) {
+      if (rv) {
+        const unsigned sq = opaque(so);
+#pragma unroll
+        for (int t = 0; t < NB; ++t) *lane_at(in.x[t] + (size_t)j * nr, sq) = fr[t];
       }
-      lds_barrier();
-      const double alpha = 2.0 * dtf * j;
-      double fr[NB];
 #pragma unroll
-      for (int t = 0; t < NB; ++t) fr[t] = pdot<S::NQ>(blk[d], &xs[k & 1][t][kq * S::CPL]);
-#pragma unroll
-      for (int t = 0; t < NB; ++t) fr[t] = quad_sum(fr[t]);
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        const double f = fr[t];
-        double* dst = (live && rv) ? in.x[t] + (size_t)j * nr + r : sink;
-        *dst = f;
-        b[t] = live ? b[t] + alpha * f : b[t];
-      }
-      fetch(k + KD, blk[d], gv[d]);
+      for (int t = 0; t < NB; ++t) b[t] = b[t] + alpha * fr[t];
     }
-  }
+    fetch(k + KD, blk[d], gv[d]);
+  };
+  int kb = k0;
+  for (; kb + KD <= k1; kb += KD) {
 #pragma unroll
-  for (int t = 0; t < NB; ++t) {
-    double* dst = rv ? in.ex[t] + (exit_seg - a.EX) + r : sink;
-    *dst = b[t];
+    for (int d = 0; d < KD; ++d) row(kb + d, d, true);
+  }
+  if (kb < k1) {
+#pragma unroll
+    for (int d = 0; d < KD; ++d) row(kb + d, d, kb + d < k1);
+  }
+  if (rv) {
+#pragma unroll
+    for (int t = 0; t < NB; ++t) *lane_at(in.ex[t] + (exit_seg - a.EX), so) = b[t];
   }
 }
 
@@ -350,17 +382,19 @@ __device__ __forceinline__ void local_a4x(const ChainArgs& a, int start, int k0,
   const double dta = a.dtf[0];
   const double ir4 = a.vecs[V_IR4A4 * S::NRP + rr];
   const double ir2 = a.vecs[V_IR2A4 * S::NRP + rr];
+  // wave-uniform row bases, lane parts as byte offsets
+  const unsigned lo = 16u * lq, go = 8u * rr, so = 8u * r;
   double2 d2[S::NQ];
-  load_blk<S::NQ>(d2, a.D2q + w * S::NQ * 128 + 2 * lq);
-  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lq;
-  double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
+  load_blk<S::NQ>(d2, lane_at(a.D2q + w * S::NQ * 128, lo));
+  const double* lb = a.Lq[0] + w * S::NQ * 128;
   double2 blk[KD][S::NQ];
   double gv[KD][NB];
   auto fetch = [&](int k, double2 (&bq)[S::NQ], double (&g)[NB]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
-    load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
+    load_blk<S::NQ>(bq, lane_at(lb + (size_t)(K - 1 - j) * S::BLK, opaque(lo)));
+    const unsigned gq = opaque(go);
 #pragma unroll
-    for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
+    for (int t = 0; t < NB; ++t) g[t] = *lane_at(in.g[t] + (size_t)j * nr, gq);
   };
 #pragma unroll
   for (int d = 0; d < KD; ++d) fetch(k0 + d, blk[d], gv[d]);
@@ -368,66 +402,77 @@ __device__ __forceinline__ void local_a4x(const ChainArgs& a, int start, int k0,
   double fe[NB], bfe[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t) fe[t] = bfe[t] = 0.0;
-  for (int kb = k0; kb < k1; kb += KD) {
+  // one row of the chain; live is false only for the rows of the last group
+  // that lie past the segment's end (uniform over the workgroup)
+  auto step_row = [&](int k, int d, bool live) {
+    const int row = start - 2 * (live ? k : k1 - 1);
+    // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt, j = row + 1:
+    // b = dt bjt (L1 f_e + IR4 bf_e) - bjt IR2 f_e with L1 = D2 + bj IR4;
+    // f = Linv (g + b); bf_e += bj f + bjt f_e; f_e += f.  f_e is also the
+    // same-parity suffix sum P of the row.  Buffer 0 carries f_e, buffer 1
+    // the right-hand side; the barrier after buffer 1 is written orders
+    // every read of buffer 0 before the next row overwrites it.
+    const double jp = (double)(row + 1);
+    const double bj = -jp * (jp + 1.0);
+    const double bjt = -2.0 * jp;
+    if (live && rv) {
+      const unsigned sq = opaque(so);
 #pragma unroll
-    for (int d = 0; d < KD; ++d) {
-      const int k = kb + d;
-      const bool live = k < k1;
-      const int row = start - 2 * (live ? k : k1 - 1);
-      // A4 (Matrix_Operators.py:1141-1164), dt = Pr dt, j = row + 1:
-      // b = dt bjt (L1 f_e + IR4 bf_e) - bjt IR2 f_e with L1 = D2 + bj IR4;
-      // f = Linv (g + b); bf_e += bj f + bjt f_e; f_e += f.  f_e is also the
-      // same-parity suffix sum P of the row.  Buffer 0 carries f_e, buffer 1
-      // the right-hand side; the barrier after buffer 1 is written orders
-      // every read of buffer 0 before the next row overwrites it.
-      const double jp = (double)(row + 1);
-      const double bj = -jp * (jp + 1.0);
-      const double bjt = -2.0 * jp;
+      for (int t = 0; t < NB; ++t) *lane_at(in.p[t] + (size_t)row * nr, sq) = fe[t];
+    }
 #pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        double* pdst = (live && rv) ? in.p[t] + (size_t)row * nr + r : sink;
-        *pdst = fe[t];
-        xs[0][t][wp] = fe[t];
+    for (int t = 0; t < NB; ++t) xs[0][t][wp] = fe[t];
+    lds_barrier();
+    // the instances' products in separate loops, so that their dependent
+    // chains interleave instead of running one after the other
+    double d2fe[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) d2fe[t] = pdot<S::NQ>(d2, &xs[0][t][kq * S::CPL]);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) d2fe[t] = quad_sum(d2fe[t]);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      const double l1fe = d2fe[t] + bj * (ir4 * fe[t]);
+      const double bb = (dta * bjt) * (l1fe + ir4 * bfe[t]) - bjt * (ir2 * fe[t]);
+      xs[1][t][wp] = gv[d][t] + bb;
+    }
+    lds_barrier();
+    double fr[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) fr[t] = pdot<S::NQ>(blk[d], &xs[1][t][kq * S::CPL]);
+#pragma unroll
+    for (int t = 0; t < NB; ++t) fr[t] = quad_sum(fr[t]);
+    if (live) {
+      if (rv) {
+        const unsigned sq = opaque(so);
+#pragma unroll
+        for (int t = 0; t < NB; ++t) *lane_at(in.x[t] + (size_t)row * nr, sq) = fr[t];
       }
-      lds_barrier();
-      // the instances' products in separate loops, so that their dependent
-      // chains interleave instead of running one after the other
-      double d2fe[NB];
-#pragma unroll
-      for (int t = 0; t < NB; ++t) d2fe[t] = pdot<S::NQ>(d2, &xs[0][t][kq * S::CPL]);
-#pragma unroll
-      for (int t = 0; t < NB; ++t) d2fe[t] = quad_sum(d2fe[t]);
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        const double l1fe = d2fe[t] + bj * (ir4 * fe[t]);
-        const double bb = (dta * bjt) * (l1fe + ir4 * bfe[t]) - bjt * (ir2 * fe[t]);
-        xs[1][t][wp] = gv[d][t] + bb;
-      }
-      lds_barrier();
-      double fr[NB];
-#pragma unroll
-      for (int t = 0; t < NB; ++t) fr[t] = pdot<S::NQ>(blk[d], &xs[1][t][kq * S::CPL]);
-#pragma unroll
-      for (int t = 0; t < NB; ++t) fr[t] = quad_sum(fr[t]);
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
         const double f = fr[t];
-        double* dst = (live && rv) ? in.x[t] + (size_t)row * nr + r : sink;
-        *dst = f;
-        const double nbfe = bfe[t] + bj * f + bjt * fe[t];
-        bfe[t] = live ? nbfe : bfe[t];
-        fe[t] = live ? fe[t] + f : fe[t];
+        bfe[t] = bfe[t] + bj * f + bjt * fe[t];
+        fe[t] = fe[t] + f;
       }
-      fetch(k + KD, blk[d], gv[d]);
     }
+    fetch(k + KD, blk[d], gv[d]);
+  };
+  int kb = k0;
+  for (; kb + KD <= k1; kb += KD) {
+#pragma unroll
+    for (int d = 0; d < KD; ++d) step_row(kb + d, d, true);
+  }
+  if (kb < k1) {
+#pragma unroll
+    for (int d = 0; d < KD; ++d) step_row(kb + d, d, kb + d < k1);
   }
   const size_t off = exit_seg - a.EX;
+  if (rv) {
 #pragma unroll
-  for (int t = 0; t < NB; ++t) {
-    double* df = rv ? in.ex[t] + off + r : sink;
-    double* db = rv ? in.ex[t] + off + S::NRP + r : sink;
-    *df = fe[t];
-    *db = bfe[t];
+    for (int t = 0; t < NB; ++t) {
+      *lane_at(in.ex[t] + off, so) = fe[t];
+      *lane_at(in.ex[t] + off + S::NRP, so) = bfe[t];
+    }
   }
 }
 
