@@ -1189,7 +1189,11 @@ static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
   hipLaunchKernelGGL((k_chain_local<NRS, NB>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local")) return rc;
   if (a.nseg > 1) {
-    hipLaunchKernelGGL((k_chain_join<NRS, NB>), grid, block, 0, s, a);
+    // four instances per join workgroup spill to scratch from NRS = 48 on;
+    // there the joins take two (the same arithmetic per instance)
+    constexpr int NBJ = (NB > 2 && NRS >= 48) ? 2 : NB;
+    const dim3 gj(chain_grid_x(a), a.nseg, (B + NBJ - 1) / NBJ);
+    hipLaunchKernelGGL((k_chain_join<NRS, NBJ>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
   }
   return SDDC_OK;
