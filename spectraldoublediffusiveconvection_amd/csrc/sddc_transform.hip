@@ -216,8 +216,15 @@ __device__ __forceinline__ void scale_products(cplx<R>* O, int L, int* red, R (&
   __syncthreads();
 }
 
+// Threads of the generic kernel.  It only runs the linearised products of
+// grids shorter than 192 points (N_fm below 128; the longer grids have the
+// in-place kernels), so four wavefronts cover every loop over the grid, and
+// at 1024 threads its scale arrays spilled to scratch (the 128-VGPR limit of
+// four wavefronts per SIMD).
+constexpr int kNlGenericThreads = 256;
+
 template <typename R, bool DFX>
-__global__ __launch_bounds__(1024) void k_nonlinear(FftShape sh, const cplx<R>* __restrict__ tw_g,
+__global__ __launch_bounds__(kNlGenericThreads) void k_nonlinear(FftShape sh, const cplx<R>* __restrict__ tw_g,
                                                    const cplx<R>* __restrict__ rot_g, int K, int nr,
                                                    const R* __restrict__ slabs,
                                                    const R* __restrict__ slabs2,
@@ -268,6 +275,10 @@ __global__ __launch_bounds__(1024) void k_nonlinear(FftShape sh, const cplx<R>* 
     is[f] = (R)1 / sc[f];
   }
 
+  // unrolled, here and below: the pair index then selects entries of the
+  // scale arrays at compile time and they stay in registers (indexed at run
+  // time they were placed in scratch memory)
+#pragma unroll
   for (int p = 0; p < 5; ++p) syn_pair<R>(A + p * L, sr, p, K, L, rot, sc);
   __syncthreads();
   C* G = fft_run<R>(A, Bf, 5, sh, tw, true);   // grid fields of X (scaled)
@@ -307,6 +318,7 @@ __global__ __launch_bounds__(1024) void k_nonlinear(FftShape sh, const cplx<R>* 
     Zq[q] = {(R)0, (R)0};
     Zt[q] = {(R)0, (R)0};
   }
+#pragma unroll
   for (int p = 0; p < 5; ++p) {
     syn_pair<R>(O, sd, p, K, L, rot, sc + 9);
     __syncthreads();
@@ -1042,7 +1054,8 @@ static int launch_nl_t(const Plan& p, int B, const void* slabs, const void* slab
     return fail(SDDC_ERR_UNSUPPORTED, "N_fm too large for the LDS nonlinear kernel");
   static std::atomic<size_t> cur{0};
   if (int rc = ensure_dynamic_lds((const void*)k_nonlinear<R, DFX>, lds, cur)) return rc;
-  hipLaunchKernelGGL((k_nonlinear<R, DFX>), dim3(p.nr, B), dim3(1024), lds, s, p.tabM.shape, tw,
+  hipLaunchKernelGGL((k_nonlinear<R, DFX>), dim3(p.nr, B), dim3(kNlGenericThreads), lds, s,
+                     p.tabM.shape, tw,
                      rot, p.K, p.nr, (const R*)slabs, (const R*)slabs2, (R*)F);
   return hip_check(hipGetLastError(), "k_nonlinear");
 }
