@@ -1189,9 +1189,12 @@ static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
   hipLaunchKernelGGL((k_chain_local<NRS, NB>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local")) return rc;
   if (a.nseg > 1) {
-    // four instances per join workgroup spill to scratch from NRS = 48 on;
-    // there the joins take two (the same arithmetic per instance)
-    constexpr int NBJ = (NB > 2 && NRS >= 48) ? 2 : NB;
+    // four instances per join workgroup need 472 VGPRs at NRS = 40 (one
+    // wavefront per SIMD) and spill to scratch from NRS = 48 on; there the
+    // joins take two (the same arithmetic per instance).  Measured on the
+    // library's default width 4 at 64 instances, N_r = 40: the solve took
+    // 111 against 121 µs at N_fm = 256 and 197 against 201 µs at 512.
+    constexpr int NBJ = (NB > 2 && NRS >= 40) ? 2 : NB;
     const dim3 gj(chain_grid_x(a), a.nseg, (B + NBJ - 1) / NBJ);
     hipLaunchKernelGGL((k_chain_join<NRS, NBJ>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;

@@ -374,14 +374,14 @@ def test_solve_widths_padded_instances(width):
     p.close()
 
 
-@pytest.mark.parametrize("K,Nr,segment", [(512, 40, 0), (64, 20, 8), (32, 65, 8)])
+@pytest.mark.parametrize("K,Nr,segment", [(512, 40, 0), (64, 20, 8), (64, 40, 8), (32, 65, 8)])
 def test_solve_width_groups_same_bits(K, Nr, segment):
     """Solve widths 1-4 give the same bits, as do 5 and 6: within a group the
     per-row arithmetic and summation order are the same, and the widths differ
     only in how many instances share a workgroup and how many rows of operator
-    data are in flight (three for width 5, two for the others).  At N_r = 65
-    width 4's segment joins take two instances per workgroup instead of four
-    (four spill to scratch there)."""
+    data are in flight (three for width 5, two for the others).  At N_r = 40
+    and 65 width 4's segment joins take two instances per workgroup instead
+    of four (four need one wavefront per SIMD or spill there)."""
     op = ops.build(K, Nr, 0.35, 5e-3, 1.0, 1 / 15.0)
     nr = Nr - 1
     rng = np.random.default_rng(K)
