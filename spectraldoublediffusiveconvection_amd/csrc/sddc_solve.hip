@@ -1321,7 +1321,12 @@ int launch_solve_ex(const Plan& p, int B, const double* G, long long gstride,
     a.fields_mask = fields_mask & 1;
     if (!a.fields_mask) return SDDC_OK;
   }
-  int NB = B >= 64 ? 4 : (B >= 8 ? 2 : 1);
+  // untuned plans: widths of one bit group (1-4 and 10 do the same
+  // arithmetic), so the results do not depend on the batch size of a call.
+  // From 64 instances width 3 (two A4 and four T/S instances per
+  // workgroup): width 4 (four of each) needs one wavefront per SIMD from
+  // N_r = 40 on and was 11-13% slower in the tuning runs at 64 and 256.
+  int NB = B >= 64 ? 3 : (B >= 8 ? 2 : 1);
   if (p.tune_solve_nbi > 0) NB = p.tune_solve_nbi;   // 16: MFMA chain kernel
   return dispatch_chain(a, p.NRS, NB, B, s);
 }
