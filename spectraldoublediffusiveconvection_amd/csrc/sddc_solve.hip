@@ -29,14 +29,18 @@
 // Rows r >= nr are padding: their matrix columns are zero, their lanes read
 // the operator entries and inputs of row nr-1 (cache lines the valid lanes
 // fetch anyway), so their values stay finite, never reach a real row through
-// the zero columns, and are not stored: the block chains of the local launch
-// mask those lanes off their stores; the other kernels send them to a sink.
-// The local block chains also run their rows in full groups of prefetched
-// rows with no per-row liveness selects (the last, partial group checks),
-// and address every load and store as a wave-uniform row base plus a fixed
-// 32-bit lane offset.  That form fits the compiler's scalar-base addressing,
-// so a row spends no vector instructions on addresses, and the kernel needs
-// 140 instead of 178 VGPRs (three wavefronts per SIMD instead of two).
+// the zero columns, and are not stored: the NAB2 chains and the
+// two-exchange A4 chains of the local launch (local_nab2, local_a4x) mask
+// those lanes off their stores; the one-exchange A4 chain (local_a4), the
+// joins and the MFMA kernel send them to a sink.  Those two chains also run
+// their rows in full groups of prefetched rows with no per-row liveness
+// selects (the last, partial group checks), and address every load and
+// store as a wave-uniform row base plus a fixed 32-bit lane offset.  That
+// form fits the compiler's scalar-base addressing, so a row spends no
+// vector instructions on addresses, and the width-5 kernel needs 140
+// instead of 178 VGPRs (three wavefronts per SIMD instead of two).  The
+// same rewrite made local_a4 slower and the joins need more registers, so
+// they keep the sink form.
 // A matrix-vector product is one partial dot product per lane (NC/4 terms)
 // and a quad reduction (two DPP exchanges, which leave the same sum in all
 // four lanes).  The vector is broadcast through LDS, stored as [c%4][c/4] so
