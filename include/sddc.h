@@ -133,7 +133,11 @@ int sddc_plan_create(const sddc_operators* ops, int precision, int symmetric,
 /* sddc_plan_create with explicit launch shapes (tuning may be NULL). */
 int sddc_plan_create_tuned(const sddc_operators* ops, int precision, int symmetric,
                            int max_batch, const sddc_tuning* tuning, sddc_plan** out);
-/* The launch shapes a plan uses (defaults resolved). */
+/* The launch shapes a plan uses (defaults resolved, except two chosen per
+ * call: solve_instances 0 is 1 below 8 instances, 2 below 64 and 3 from 64
+ * on, all of one bit group, so a plan's results do not depend on the
+ * batch size of a call; spectral_instances 0 follows the call's number of
+ * (mode tile, instance) pairs, any value giving the same bits). */
 int sddc_plan_tuning(const sddc_plan* plan, sddc_tuning* out);
 /* Grow the plan's workspace to max_batch instances (no-op if it holds that
  * many).  Growing frees the old workspace, so it is refused with
