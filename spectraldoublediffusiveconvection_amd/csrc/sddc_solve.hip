@@ -38,9 +38,10 @@
 // store as a wave-uniform row base plus a fixed 32-bit lane offset.  That
 // form fits the compiler's scalar-base addressing, so a row spends no
 // vector instructions on addresses, and the width-5 kernel needs 140
-// instead of 178 VGPRs (three wavefronts per SIMD instead of two).  The
-// same rewrite made local_a4 slower and the joins need more registers, so
-// they keep the sink form.
+// instead of 178 VGPRs (three wavefronts per SIMD instead of two).  In
+// local_a4 only the addressing pays (masked stores and peeled groups made
+// its rows slower), and the joins need more registers either way, so those
+// keep the sink form.
 // A matrix-vector product is one partial dot product per lane (NC/4 terms)
 // and a quad reduction (two DPP exchanges, which leave the same sum in all
 // four lanes).  The vector is broadcast through LDS, stored as [c%4][c/4] so
@@ -286,17 +287,19 @@ __device__ __forceinline__ void local_a4(const ChainArgs& a, int start, int k0, 
   const double dta = a.dtf[0];
   const double ir4 = a.vecs[V_IR4A4 * S::NRP + rr];
   const double ir2 = a.vecs[V_IR2A4 * S::NRP + rr];
-  const double* lb = a.Lq[0] + w * S::NQ * 128 + 2 * lq;
-  const double* mb = a.DLq + w * S::NQ * 128 + 2 * lq;
+  const double* lb = a.Lq[0] + w * S::NQ * 128;
+  const double* mb = a.DLq + w * S::NQ * 128;
+  const unsigned lo = 16u * lq, go = 8u * rr;
   double* sink = a.scratch + 2 * (size_t)K * nr + threadIdx.x;
   double2 blk[KD][S::NQ], mblk[KD][S::NQ];
   double gv[KD][NB];
   auto fetch = [&](int k, double2 (&bq)[S::NQ], double2 (&mq)[S::NQ], double (&g)[NB]) {
     const int j = start - 2 * (k < k1 ? k : k1 - 1);
-    load_blk<S::NQ>(bq, lb + (size_t)(K - 1 - j) * S::BLK);
-    load_blk<S::NQ>(mq, mb + (size_t)(K - 1 - j) * S::BLK);
+    load_blk<S::NQ>(bq, lane_at(lb + (size_t)(K - 1 - j) * S::BLK, opaque(lo)));
+    load_blk<S::NQ>(mq, lane_at(mb + (size_t)(K - 1 - j) * S::BLK, opaque(lo)));
+    const unsigned gq = opaque(go);
 #pragma unroll
-    for (int t = 0; t < NB; ++t) g[t] = in.g[t][(size_t)j * nr + rr];
+    for (int t = 0; t < NB; ++t) g[t] = *lane_at(in.g[t] + (size_t)j * nr, gq);
   };
 #pragma unroll
   for (int d = 0; d < KD; ++d) fetch(k0 + d, blk[d], mblk[d], gv[d]);
