@@ -104,7 +104,7 @@ __host__ __device__ constexpr int spectral_units() {
 }
 
 template <bool kMixed, int NT>
-__device__ __forceinline__ int spectral_unit(int w, int j) {
+__host__ __device__ constexpr int spectral_unit(int w, int j) {
   if (!kMixed) return j < NT ? (w << 3) | j : -1;
   constexpr int U = spectral_units<kMixed, NT>();
   // rows: wavefronts; entries: (c << 3) | it, -1 = none
@@ -120,8 +120,30 @@ __device__ __forceinline__ int spectral_unit(int w, int j) {
   else return t4[w][j];
 }
 
+// In mixed plans only a wavefront's first unit may be the float64
+// contraction; the others are float and load their tiles as floats.
+template <int NT>
+constexpr bool mixed_units_after_first_are_float() {
+  for (int w = 0; w < 5; ++w)
+    for (int j = 1; j < spectral_units<true, NT>(); ++j)
+      if (spectral_unit<true, NT>(w, j) >= 0 && (spectral_unit<true, NT>(w, j) >> 3) == 2)
+        return false;
+  return true;
+}
+
+// Waves per SIMD the register budget must allow: the mixed 48-row kernel's
+// LDS (about 51 KB) fits three workgroups of five wavefronts per CU, which
+// needs four waves per SIMD (at most 128 VGPRs); left free, the compiler
+// took 129.
 template <typename R, int NRP>
-__global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
+constexpr int spectral_waves_per_simd() {
+  return std::is_same<R, float>::value && NRP == 48 ? 4 : 1;
+}
+
+template <typename R, int NRP>
+__global__ __launch_bounds__(320)
+__attribute__((amdgpu_waves_per_eu(spectral_waves_per_simd<R, NRP>()))) void k_spectral(
+    SpectralArgs a) {
   constexpr int NT = NRP / 16;
   constexpr int NK = NRP / 4;
   // rows padded to an odd number of doubles: the MFMA operand reads take one
@@ -160,23 +182,29 @@ __global__ __launch_bounds__(320) void k_spectral(SpectralArgs a) {
   double a0[NK];                                    // unit 0 (float64 or float)
   float af[kUnits > 1 ? kUnits - 1 : 1][NK];        // units 1.. (mixed: float)
   double ad[kMixed ? 1 : (kUnits > 1 ? kUnits - 1 : 1)][NK];   // units 1.. (float64 plan)
+  static_assert(!kMixed || mixed_units_after_first_are_float<NT>(),
+                "mixed plans hold units after the first as floats");
+  // Whether a unit is float64 depends on the wavefront index, known only at
+  // run time, so the loads carry no per-unit choice of array or conversion
+  // (either would make each load wait before the next is issued): unit 0
+  // always loads its float64 tile (rounded to float where it is used, which
+  // gives the float array's value: that array holds the same operators
+  // rounded to float), the float units load the float array.
 #pragma unroll
   for (int j = 0; j < kUnits; ++j) {
     const int code = spectral_unit<kMixed, NT>(w, j);
     if (code < 0) continue;
     const int cc = code >> 3, it = code & 7;
     const int slot = (cc == 1) ? 1 : (cc == 2 ? 2 : 0);   // Dr, D2r2, D2s
-    const bool f64 = !kMixed || cc == 2;
     const size_t base = (size_t)slot * NT * NK * 64 + (size_t)it * NK * 64 + lane;
 #pragma unroll
     for (int kk = 0; kk < NK; ++kk) {
-      const double v = f64 ? a.A64[base + kk * 64] : (double)a.A32[base + kk * 64];
       if (j == 0)
-        a0[kk] = v;
+        a0[kk] = a.A64[base + kk * 64];
       else if (kMixed)
-        af[j - 1][kk] = (float)v;
+        af[j - 1][kk] = a.A32[base + kk * 64];
       else
-        ad[kMixed ? 0 : j - 1][kk] = v;
+        ad[kMixed ? 0 : j - 1][kk] = a.A64[base + kk * 64];
     }
   }
   constexpr int kVecIters = (kVecs * NRP + 319) / 320;
@@ -528,6 +556,23 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
   if (a.have_nonlin) {
     const R* Fb = reinterpret_cast<const R*>(a.F) + (size_t)b * 4 * nK;
     constexpr int kIters = (NRP * kTile + 255) / 256;
+    // every lane loads (entry 0 where out of range) before any value is
+    // selected or stored: a load under its range condition would be waited
+    // for before the next one is issued
+    R fv[kIters][4];
+#pragma unroll
+    for (int it = 0; it < kIters; ++it) {
+      const int idx = threadIdx.x + 256 * it;
+      const int i = idx / kTile, rl = idx - i * kTile;
+      const int r = k0 + rl, m = r + 1;
+      const bool in = idx < NRP * kTile && i < nr;
+      const size_t row = (size_t)i * K;
+      const size_t om = (in && m < K) ? row + m : 0, orr = (in && r < K) ? row + r : 0;
+      fv[it][0] = Fb[om];
+      fv[it][1] = Fb[nK + om];
+      fv[it][2] = Fb[2 * nK + orr];
+      fv[it][3] = Fb[3 * nK + orr];
+    }
 #pragma unroll
     for (int it = 0; it < kIters; ++it) {
       const int idx = threadIdx.x + 256 * it;
@@ -535,11 +580,10 @@ __global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
         const int i = idx / kTile, rl = idx - i * kTile;
         const int r = k0 + rl, m = r + 1;
         const bool okm = (i < nr && m < K), okr = (i < nr && r < K);
-        const size_t row = (size_t)i * K;
-        sQ[rl][i] = okm ? Fb[row + m] : (R)0;
-        sR[rl][i] = okm ? Fb[nK + row + m] : (R)0;
-        sTT[rl][i] = okr ? Fb[2 * nK + row + r] : (R)0;
-        sSS[rl][i] = okr ? Fb[3 * nK + row + r] : (R)0;
+        sQ[rl][i] = okm ? fv[it][0] : (R)0;
+        sR[rl][i] = okm ? fv[it][1] : (R)0;
+        sTT[rl][i] = okr ? fv[it][2] : (R)0;
+        sSS[rl][i] = okr ? fv[it][3] : (R)0;
       }
     }
     __syncthreads();
