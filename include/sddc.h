@@ -102,10 +102,10 @@ typedef struct sddc_plan sddc_plan;
  *                    different orders (equal to rounding), so the choice is
  *                    fixed per plan
  *   sub_batches      graphs only: the batch is split into this many parts
- *                    (1-8), each stepped on its own forked stream inside the
- *                    graph, so that the latency-bound solves of one part
- *                    overlap the transforms of another.  Results are the
- *                    same bits as one batch.
+ *                    (1-8), stepped on forked streams inside the graph (see
+ *                    sddc_graph_create_multi), so that the latency-bound
+ *                    solves of one part overlap the transforms of another.
+ *                    Results are the same bits as one batch.
  *   nab2_solve       the T and S solves: 0 or 1 chains of per-mode block
  *                    inverses (the reference's form), 2 scalar recurrences
  *                    in the eigenbasis of the operator set's modes_T /
@@ -221,10 +221,15 @@ typedef struct sddc_graph sddc_graph;
 int sddc_graph_create(sddc_plan* plan, int B, const double* Ra, const double* Ra_s,
                       double* X, int nsteps, void* stream, sddc_graph** out);
 /* One graph for several size classes: nsteps in-place steps of batch
- * X[c] (B[c] instances, parameters Ra[c], Ra_s[c]) of every plan c.  Each
- * class runs on its own forked stream inside the graph, so classes execute
- * concurrently.  No counterpart in the reference: batches parameter sweeps
- * over mixed resolutions into one launch. */
+ * X[c] (B[c] instances, parameters Ra[c], Ra_s[c]) of every plan c.  The
+ * parts (each class split by its plan's sub_batches) run concurrently on
+ * forked streams inside the graph, at most one per hardware queue
+ * (GPU_MAX_HW_QUEUES, 4 when unset): with more parts than that, each part
+ * is timed alone for one step at creation (into the plan's spare state
+ * rows; X is not changed) and the parts are spread over the streams by that
+ * time, longest first onto the least loaded, each stream running its parts
+ * in turn.  No counterpart in the reference: batches parameter sweeps over
+ * mixed resolutions into one launch. */
 int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
                             const double* const* Ra, const double* const* Ra_s,
                             double* const* X, int nsteps, void* stream, sddc_graph** out);

@@ -1,7 +1,9 @@
 // C ABI of libsddc: plan lifecycle, operator upload and the entry points
 // declared in include/sddc.h.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -620,6 +622,14 @@ static Plan static_view(const Plan& p, int B, unsigned char* base, size_t* need)
   return v;
 }
 
+// Hardware queues the HIP runtime gives this process (GPU_MAX_HW_QUEUES,
+// 4 when unset).
+static int hw_queue_count() {
+  const char* e = getenv("GPU_MAX_HW_QUEUES");
+  const int n = e ? atoi(e) : 0;
+  return n >= 1 ? n : 4;
+}
+
 // Parts [b0, b0 + nb) of a batch of B for `parts` forked streams: sizes are
 // multiples of 16 (the last part takes the rest); fewer parts when B is small.
 static std::vector<std::pair<int, int>> split_batch(int B, int parts) {
@@ -767,35 +777,80 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
     for (auto [b0, nb] : split_batch(B[c], plans[c]->p.tune_sub_batches))
       parts.push_back({sub_plan(plans[c]->p, b0, nb), c, b0, nb});
   const int np = (int)parts.size();
-  g->side.assign(np, nullptr);
-  g->events.assign(np + 1, nullptr);
-  for (int q = 0; q < np; ++q)
-    if (hipStreamCreateWithFlags(&g->side[q], hipStreamNonBlocking) != hipSuccess)
-      return bail(fail(SDDC_ERR_HIP, "stream creation failed"));
-  for (auto& e : g->events)
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-      return bail(fail(SDDC_ERR_HIP, "event creation failed"));
+  auto state_of = [&](const Part& pt) {
+    return X[pt.c] + pt.b0 * (size_t)3 * pt.view.K * pt.view.nr;
+  };
   // one eager step per class so that every lazily set kernel attribute is in place
   int rc = SDDC_OK;
   for (int c = 0; c < nplans && !rc; ++c)
     rc = sddc_step(plans[c], B[c], Ra[c], Ra_s[c], X[c], plans[c]->p.Xb, 1, 0, s);
   if (!rc) rc = hip_check(hipStreamSynchronize(s), "warm-up");
   if (rc) return bail(rc);
-  // fork: every part's stream waits for the capture origin, runs its steps,
-  // and the origin waits for all of them
+  // Lanes.  Kernels on one hardware queue run one after another, and a
+  // process has few queues, so with more parts than queues the parts that
+  // share a queue add up, however the runtime places the graph's branches.
+  // With more parts than queues, each part is timed alone (one step, written
+  // to the plan's spare state rows; the second of two runs, so that no
+  // first launch is counted) and the parts go onto as many lanes as there
+  // are queues, longest first onto the least loaded; each lane is one forked
+  // capture stream that runs its parts in turn.
+  const int nl = std::min(np, hw_queue_count());
+  std::vector<int> lane(np);
+  for (int q = 0; q < np; ++q) lane[q] = q;
+  if (nl < np) {
+    std::vector<float> cost(np, 0.f);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    rc = hip_check(hipEventCreate(&t0), "event creation");
+    if (!rc) rc = hip_check(hipEventCreate(&t1), "event creation");
+    for (int q = 0; q < np && !rc; ++q) {
+      Part& pt = parts[q];
+      for (int rep = 0; rep < 2 && !rc; ++rep) {
+        rc = hip_check(hipEventRecord(t0, s), "timing");
+        if (!rc)
+          rc = run_steps(pt.view, pt.nb, Ra[pt.c] + pt.b0, Ra_s[pt.c] + pt.b0, state_of(pt),
+                         pt.view.Xb, 1, 0, s);
+        if (!rc) rc = hip_check(hipEventRecord(t1, s), "timing");
+        if (!rc) rc = hip_check(hipEventSynchronize(t1), "timing");
+        if (!rc) rc = hip_check(hipEventElapsedTime(&cost[q], t0, t1), "timing");
+      }
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (rc) return bail(rc);
+    std::vector<int> order(np);
+    for (int q = 0; q < np; ++q) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<float> load(nl, 0.f);
+    for (int q : order) {
+      const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      lane[q] = l;
+      load[l] += cost[q];
+    }
+  }
+  g->side.assign(nl, nullptr);
+  g->events.assign(nl + 1, nullptr);
+  for (int l = 0; l < nl; ++l)
+    if (hipStreamCreateWithFlags(&g->side[l], hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(SDDC_ERR_HIP, "stream creation failed"));
+  for (auto& e : g->events)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(SDDC_ERR_HIP, "event creation failed"));
+  // fork: every lane's stream waits for the capture origin, runs its parts'
+  // steps, and the origin waits for all of them
   if ((rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "capture")))
     return bail(rc);
-  int rc2 = hip_check(hipEventRecord(g->events[np], s), "fork");
-  for (int q = 0; q < np && !rc2; ++q) {
-    Part& pt = parts[q];
-    const size_t n3 = (size_t)3 * pt.view.K * pt.view.nr;
-    double* xq = X[pt.c] + pt.b0 * n3;
-    rc2 = hip_check(hipStreamWaitEvent(g->side[q], g->events[np], 0), "fork wait");
-    if (!rc2)
+  int rc2 = hip_check(hipEventRecord(g->events[nl], s), "fork");
+  for (int l = 0; l < nl && !rc2; ++l) {
+    rc2 = hip_check(hipStreamWaitEvent(g->side[l], g->events[nl], 0), "fork wait");
+    for (int q = 0; q < np && !rc2; ++q) {
+      if (lane[q] != l) continue;
+      Part& pt = parts[q];
+      double* xq = state_of(pt);
       rc2 = run_steps(pt.view, pt.nb, Ra[pt.c] + pt.b0, Ra_s[pt.c] + pt.b0, xq, xq, nsteps, 0,
-                      g->side[q]);
-    if (!rc2) rc2 = hip_check(hipEventRecord(g->events[q], g->side[q]), "join");
-    if (!rc2) rc2 = hip_check(hipStreamWaitEvent(s, g->events[q], 0), "join wait");
+                      g->side[l]);
+    }
+    if (!rc2) rc2 = hip_check(hipEventRecord(g->events[l], g->side[l]), "join");
+    if (!rc2) rc2 = hip_check(hipStreamWaitEvent(s, g->events[l], 0), "join wait");
   }
   hipError_t e = hipStreamEndCapture(s, &g->graph);
   rc = rc2 ? rc2 : hip_check(e, "end capture");
