@@ -164,14 +164,16 @@ struct Inst {
   double* ex[NB];   // exit states of segment 0 of this (field, chain)
 };
 
+// The instances of instance group `group` (NB instances per workgroup).
 template <int NB>
-__device__ __forceinline__ Inst<NB> instances(const ChainArgs& a, int field, int chain) {
+__device__ __forceinline__ Inst<NB> instances(const ChainArgs& a, int field, int chain,
+                                              int group) {
   Inst<NB> in;
   const size_t N = (size_t)a.K * a.nr;
   const size_t exs = (size_t)a.nseg * 2 * (((a.nr + 15) / 16) * 16);
 #pragma unroll
   for (int t = 0; t < NB; ++t) {
-    const int b = blockIdx.z * NB + t;
+    const int b = group * NB + t;
     const bool valid = b < a.B;
     const int bc = valid ? b : a.B - 1;
     in.g[t] = a.G + (size_t)bc * a.gstride + a.goff[field];
@@ -494,7 +496,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs 
   const int k0 = seg * a.m;
   if (k0 >= n) return;
   const int k1 = k0 + a.m < n ? k0 + a.m : n;
-  const Inst<NB> in = instances<NB>(a, field, chain);
+  const Inst<NB> in = instances<NB>(a, field, chain, blockIdx.z);
   // Eq_SYM keeps odd psi rows (A4 chain from K-1) and even T/S rows (NAB2
   // chain from K-2); the other chain is zero (Matrix_Operators.py:1054,1166).
   if (a.symmetric && (field == 0 ? chain == 1 : chain == 0)) {
@@ -534,14 +536,62 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local(ChainArgs 
 // width 6 slower and KD 4 made width 5 slower).  Four A4 or eight NAB2
 // instances per workgroup were slower (more per-row work on three
 // wavefronts), as were three NAB2 instances or one A4 instance with KD 3.
+// Work items of a k_chain_local_mixed launch, which is one dimensional and
+// holds only live items: per (field, chain) type, nseg segments of
+// ceil(B / width) instance groups (width NBA for the A4 chains, NBN for the
+// T and S chains); types in order, then segments, then groups.  Workgroups
+// go to the eight XCDs round-robin (workgroup w to XCD w % 8), so the list is
+// cut into eight runs of consecutive items and run c is given to the
+// workgroups of XCD c: each XCD's L2 then streams the operator blocks of one
+// or two (type, segment)s.  In (type, segment, group) grid order four XCDs
+// fetched every type's blocks (the solve read 1.7x its bytes).
+struct MixedItem {
+  int field, chain, seg, group;   // field -1: padding past the list
+};
+
+__host__ __device__ inline int mixed_types(int fields_mask) { return fields_mask == 1 ? 2 : 6; }
+
+__host__ __device__ inline int mixed_item_count(int types, int nseg, int za, int zn) {
+  return nseg * (2 * za + (types - 2) * zn);
+}
+
+__host__ __device__ inline int mixed_grid(int types, int nseg, int za, int zn) {
+  return (mixed_item_count(types, nseg, za, zn) + 7) / 8 * 8;
+}
+
+__device__ __forceinline__ MixedItem mixed_item(int w, int types, int nseg, int za, int zn) {
+  const int total = mixed_item_count(types, nseg, za, zn);
+  const int i = (w % 8) * (mixed_grid(types, nseg, za, zn) / 8) + w / 8;
+  MixedItem it{-1, 0, 0, 0};
+  if (i >= total) return it;
+  const int na = nseg * za;
+  int type, r, z;
+  if (i < 2 * na) {
+    type = i / na;
+    r = i - type * na;
+    z = za;
+  } else {
+    const int nn = nseg * zn, j = i - 2 * na;
+    type = 2 + j / nn;
+    r = j - (type - 2) * nn;
+    z = zn;
+  }
+  it.field = type >> 1;
+  it.chain = type & 1;
+  it.seg = r / z;
+  it.group = r - it.seg * z;
+  return it;
+}
+
 template <int NRS, int NBA, int NBN, bool X2, int KD>
 __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(ChainArgs a) {
   using S = CShape<NRS>;
   constexpr int NBX = NBA > NBN ? NBA : NBN;
   __shared__ __align__(16) double xs[2][NBX][NRS + 64];
-  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y;
-  if (!((a.fields_mask >> field) & 1)) return;
-  if ((int)blockIdx.z * (field == 0 ? NBA : NBN) >= a.B) return;
+  const MixedItem it = mixed_item(blockIdx.x, mixed_types(a.fields_mask), a.nseg,
+                                  (a.B + NBA - 1) / NBA, (a.B + NBN - 1) / NBN);
+  const int field = it.field, chain = it.chain, seg = it.seg;
+  if (field < 0 || !((a.fields_mask >> field) & 1)) return;
   const int start = a.K - 1 - chain;
   const int n = start / 2 + 1;
   const int k0 = seg * a.m;
@@ -550,7 +600,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
   double* exit_seg = a.EX + (size_t)seg * 2 * S::NRP;
   const bool sym_zero = a.symmetric && (field == 0 ? chain == 1 : chain == 0);
   if (field == 0) {
-    const Inst<NBA> in = instances<NBA>(a, field, chain);
+    const Inst<NBA> in = instances<NBA>(a, field, chain, it.group);
     if (sym_zero) {   // Eq_SYM: this chain is zero (Matrix_Operators.py:1166)
       const int lane = threadIdx.x & 63;
       const int r = 16 * (threadIdx.x >> 6) + (lane >> 2);
@@ -572,7 +622,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_local_mixed(Chai
       local_a4<NRS, NBA, KD>(a, start, k0, k1, in, xa, exit_seg);
     return;
   }
-  const Inst<NBN> in = instances<NBN>(a, field, chain);
+  const Inst<NBN> in = instances<NBN>(a, field, chain, it.group);
   if (sym_zero) {   // Eq_SYM: this chain is zero (Matrix_Operators.py:1054)
     const int lane = threadIdx.x & 63;
     const int r = 16 * (threadIdx.x >> 6) + (lane >> 2);
@@ -806,7 +856,7 @@ __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_join(ChainArgs a
   const int k0 = seg * a.m;
   if (k0 >= n) return;
   const int k1 = k0 + a.m < n ? k0 + a.m : n;
-  const Inst<NB> in = instances<NB>(a, field, chain);
+  const Inst<NB> in = instances<NB>(a, field, chain, blockIdx.z);
   if (field == 0)
     join_a4<NRS, NB>(a, chain, start, seg, k0, k1, in, xs);
   else
@@ -1212,8 +1262,9 @@ static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
 template <int NRS, int NBA, int NBN, bool X2, int KD>
 static int launch_chain_mixed_t(const ChainArgs& a, int B, hipStream_t s) {
   using S = CShape<NRS>;
-  constexpr int NBMIN = NBA < NBN ? NBA : NBN;
-  const dim3 grid(chain_grid_x(a), a.nseg, (B + NBMIN - 1) / NBMIN), block(64 * S::WR);
+  const dim3 grid(mixed_grid(mixed_types(a.fields_mask), a.nseg, (B + NBA - 1) / NBA,
+                             (B + NBN - 1) / NBN)),
+      block(64 * S::WR);
   hipLaunchKernelGGL((k_chain_local_mixed<NRS, NBA, NBN, X2, KD>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local_mixed")) return rc;
   if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup

@@ -63,6 +63,34 @@ def test_sweep_graph_matches_eager_and_alone():
 
 
 @pytest.mark.gpu
+def test_sweep_graph_with_more_parts_than_queues_matches_eager():
+    """More parts than hardware queues: the graph places the parts on lanes
+    by their timed cost, each lane running its parts in turn.  Two launches
+    of a five-step graph give the same bits as the same steps taken eagerly,
+    class by class."""
+    import os
+
+    from spectraldoublediffusiveconvection_amd import batching
+    insts = batching.random_sweep(30, seed=11, N_r=(12, 20), N_fm=(16, 32, 64))
+    tuning = {"chain_segment": 8, "solve_instances": 2, "sub_batches": 1}
+
+    class Fixed:
+        def lookup(self, key, B, precision):
+            return tuning
+    a = batching.Sweep(insts, precision="mixed", tuning_table=Fixed())
+    b = batching.Sweep(insts, precision="mixed", tuning_table=Fixed())
+    queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) or 4
+    assert len(a.classes) > queues, (len(a.classes), queues)
+    g = a.graph(nsteps=5)
+    for _ in range(2):
+        g.launch()
+        b.step(5)
+    for x, y in zip(a.states(), b.states()):
+        assert np.array_equal(x, y)
+    g.close()
+
+
+@pytest.mark.gpu
 def test_tuned_plans_agree_with_oracle():
     """Every launch shape the autotuner may pick gives the oracle's step."""
     import torch
