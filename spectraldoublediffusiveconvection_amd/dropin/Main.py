@@ -10,8 +10,9 @@ step and linearised step instead (the reference's own Krylov method).  Both
 converge to the same steady state within the Newton tolerance.  The
 continuation drivers (_NewtonC, _ContinC, _Continuation) run their Newton and
 pseudo-arclength steps on the device (sddc_newton, sddc_arclength_step).
-Checkpoints use h5py with the reference's groups when h5py is importable,
-otherwise a NumPy .npz beside the requested name.
+Checkpoints are HDF5 files with the reference's groups, written by h5py when
+it is importable and otherwise by the package's own writer (_hdf5.write:
+the same groups and datasets, readable by h5py).
 """
 from __future__ import annotations
 
@@ -22,6 +23,7 @@ import torch
 
 from .. import operators as _ops
 from . import Matrix_Operators as _mo
+from . import _hdf5
 from . import _registry as _reg
 
 
@@ -110,11 +112,9 @@ def _write_checkpoint(save_filename, X_DATA, scalars, params):
             for k, v in params.items():
                 g[k] = v
         return save_filename
-    data = {"Checkpoints/X_DATA": np.asarray(X_DATA)}
-    data.update({"Scalar_Data/" + k: np.asarray(v) for k, v in scalars.items()})
-    data.update({"Parameters/" + k: np.asarray(v) for k, v in params.items()})
-    np.savez(save_filename + ".npz", **data)
-    return save_filename + ".npz"
+    return _hdf5.write(save_filename, {"Checkpoints": {"X_DATA": X_DATA},
+                                       "Scalar_Data": dict(scalars),
+                                       "Parameters": dict(params)})
 
 
 def _Time_Step(X, Ra, Ra_s, Tau, Pr, d, N_fm, N_r, symmetric, save_filename='TimeStep_0.h5',
@@ -345,12 +345,10 @@ def _write_continuation(filename, Result, params):
             for k, v in vars(Result).items():
                 g.create_dataset(k, data=v)
         return filename
-    data = {"Checkpoints/X_DATA": np.asarray(Result.X_DATA),
-            "Checkpoints/Ra_DATA": np.asarray(Result.Ra_DATA)}
-    data.update({"Parameters/" + k: np.asarray(v) for k, v in params.items()})
-    data.update({"Bifurcation/" + k: np.asarray(v) for k, v in vars(Result).items()})
-    np.savez(filename + ".npz", **data)
-    return filename + ".npz"
+    return _hdf5.write(filename, {"Checkpoints": {"X_DATA": Result.X_DATA,
+                                                  "Ra_DATA": Result.Ra_DATA},
+                                  "Parameters": dict(params),
+                                  "Bifurcation": dict(vars(Result))})
 
 
 def _Continuation(filename, N_steps, sign, Y, **kwargs):

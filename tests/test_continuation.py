@@ -124,7 +124,7 @@ def test_dropin_continuation_driver(case, tmp_path):
     res = Main._Continuation(fn, 3, 1.0, Y0, **kw)
     assert res.Iterations == 3 and len(res.Ra) == 3
     assert case["Ra"] < res.Ra[0] < res.Ra[1] < res.Ra[2]
-    assert os.path.exists(fn) or os.path.exists(fn + ".npz")
+    assert os.path.exists(fn)
     # every branch point is a steady state
     out = Main._ContinC(0 * Y0, Y0, 1.0, 0.01, **kw)
     assert len(out) == 9 and out[-1] is True

@@ -205,14 +205,17 @@ def test_time_step_driver_vs_golden(mods, tmp_path, sym):
                          C1["d"], K, C1["N_r"], sym, save_filename=fname, start_time=0,
                          Total_time=20 * C1["dt"], dt=C1["dt"])
     assert rel(Xn, GOLD["traj20_X_" + tag]) < 1e-13
-    path = fname if os.path.exists(fname) and not os.path.exists(fname + ".npz") else fname + ".npz"
-    if path.endswith(".npz"):
-        f = np.load(path)
-        sd = {k: f["Scalar_Data/" + k] for k in ("Norm", "KE", "Nu_T", "Nu_S")}
-        assert f["Checkpoints/X_DATA"].shape[1] == 3 * N
-    else:
+    try:
         import h5py
-        with h5py.File(path, "r") as h:
+    except ImportError:
+        h5py = None
+    if h5py is None:   # the package's own HDF5 writer made the file
+        from spectraldoublediffusiveconvection_amd.dropin import _hdf5
+        f = _hdf5.read(fname)
+        sd = {k: f["Scalar_Data"][k] for k in ("Norm", "KE", "Nu_T", "Nu_S")}
+        assert f["Checkpoints"]["X_DATA"].shape[1] == 3 * N
+    else:
+        with h5py.File(fname, "r") as h:
             sd = {k: np.asarray(h["Scalar_Data"][k]) for k in ("Norm", "KE", "Nu_T", "Nu_S")}
     for k, v in sd.items():
         n = len(v)
