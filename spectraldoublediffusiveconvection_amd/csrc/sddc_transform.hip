@@ -805,31 +805,40 @@ __device__ __forceinline__ void ip_forward_project(cplx<R> (&o)[NC], cplx<R>* O,
   }
 }
 
+// Waves per SIMD the float kernel's registers must allow: its LDS (five
+// pair buffers, 31 KB at L = 768) fits five workgroups of five wavefronts per
+// CU, so a SIMD holds up to seven waves (at most 72 VGPRs; left free, the
+// compiler took 75, which kept four workgroups per CU).
+template <typename R>
+constexpr int nl_ip_waves_per_simd() {
+  return std::is_same<R, float>::value ? 7 : 1;
+}
+
 // F(X, X): five wavefronts, wavefront p synthesises field pair p.
-// LDS: twiddles (L) and five pair buffers of L values each;
-// once every product is in registers, wavefronts 0 and 1 reuse pair
-// buffers 3 and 4 for their forward transforms.
+// LDS: five pair buffers of L values each; the twiddles are read from global
+// memory (the per-pass table, L1-resident), which leaves room for a fifth
+// workgroup per CU (four with them in LDS; measured at L = 768, 64
+// instances: 48.0 -> 45.8 us in float, 429 -> 416 us in float64).  Once
+// every product is in registers, wavefronts 0 and 1 reuse pair buffers 3
+// and 4 for their forward transforms.
 template <typename R, int NC>
-__global__ __launch_bounds__(320) void k_nonlinear_ip(FftShape sh, const cplx<R>* __restrict__ tw_g,
-                                                      const cplx<R>* __restrict__ rot_g, int nr,
-                                                      const R* __restrict__ slabs,
-                                                      R* __restrict__ F) {
+__global__ __launch_bounds__(320)
+__attribute__((amdgpu_waves_per_eu(nl_ip_waves_per_simd<R>()))) void k_nonlinear_ip(
+    FftShape sh, const cplx<R>* __restrict__ tw_g, const cplx<R>* __restrict__ rot_g, int nr,
+    const R* __restrict__ slabs, R* __restrict__ F) {
   constexpr int L = 64 * NC, NK = 2 * NC / 3, K = 64 * NK;
   static_assert(NC % 3 == 0, "L = 3K/2 with K a multiple of 64");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ R sinv[10];   // unscaling factors of the nine fields
   using C = cplx<R>;
-  C* tw = reinterpret_cast<C*>(smem_raw);
-  C* buf = tw + L;
+  C* buf = reinterpret_cast<C*>(smem_raw);
+  const C* tw = tw_g;
   const int p = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int i = blockIdx.x, b = blockIdx.y;
   const SlabRows<R> sg = slab_rows<R>(slabs, b, i, nr, K);
-  IpTwiddles<R, NC, 320> twr;
-  twr.load(tw_g);
   C* A = buf + (size_t)p * L;
   ip_pack_pair<R, NC>(sg, p, A, rot_g, sinv + 2 * p, lane);
-  twr.store(tw);
-  __syncthreads();   // twiddles in LDS
+  __syncthreads();   // the five transforms start together
   fft_run_ct<R, L>(A, tw, true, lane);
   __syncthreads();
 
@@ -987,13 +996,14 @@ int ensure_dynamic_lds(const void* fn, size_t bytes, std::atomic<size_t>& curren
 // buffers (five for F(X, X), ten for the Frechet derivative).
 template <typename R>
 static size_t nl_ip_lds(int L, bool dfx = false) {
-  return ((size_t)L + (dfx ? 10 : 5) * (size_t)L) * sizeof(cplx<R>);
+  // F(X, X): five pair buffers; F(X, dv) + F(dv, X): ten, and the twiddles
+  return (dfx ? 11 : 5) * (size_t)L * sizeof(cplx<R>);
 }
 
 int nonlinear_lds_bytes(const Plan& p, int precision, int /*dfx*/) {
   const size_t c = precision == SDDC_MIXED ? sizeof(float2) : sizeof(double2);
-  // in place: twiddles + 5 pair buffers (the products reuse two); else 2 x 5
-  // working arrays + twiddles + rotations
+  // in place: 5 pair buffers (the products reuse two); else 2 x 5 working
+  // arrays + twiddles + rotations
   if (nl_inplace_nc(p))
     return (int)(precision == SDDC_MIXED ? nl_ip_lds<float>(p.M) : nl_ip_lds<double>(p.M));
   return (int)(12 * (size_t)p.M * c);
