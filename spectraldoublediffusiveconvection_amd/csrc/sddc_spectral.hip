@@ -505,8 +505,18 @@ __device__ __forceinline__ int xcd_grouped_tile(int x, int tiles) {
   return (tiles % 8 == 0) ? (x % 8) * (tiles / 8) + x / 8 : x;
 }
 
+// Waves per SIMD the mixed 48-row kernel's registers must allow: its tile
+// loads all in flight took it to 76 VGPRs (six workgroups per CU); bounded to
+// 72 (70) it keeps seven, 16.8 -> 16.4 us per 64-instance launch, same bits.
+// The 64-row kernel would spill under the same bound.
 template <typename R, int NRP>
-__global__ __launch_bounds__(256) void k_rhs(RhsArgs a) {
+constexpr int rhs_waves_per_simd() {
+  return std::is_same<R, float>::value && NRP == 48 ? 7 : 1;
+}
+
+template <typename R, int NRP>
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(rhs_waves_per_simd<R, NRP>()))) void k_rhs(RhsArgs a) {
   constexpr int NT = NRP / 16;   // <= 4: one row tile per wavefront
   constexpr int NK = NRP / 4;
   // rows padded to an odd length: the transposing fill below and the MFMA
