@@ -767,7 +767,7 @@ int sddc_graph_create_multi(int nplans, sddc_plan* const* plans, const int* B,
       return bail(fail(SDDC_ERR_HIP, "stream creation failed"));
     s = g->own;
   }
-  // one forked stream per part: each class split into its plan's sub_batches
+  // the parts: each class split into its plan's sub_batches
   struct Part {
     Plan view;
     int c, b0, nb;
