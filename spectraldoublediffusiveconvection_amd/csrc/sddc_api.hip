@@ -20,7 +20,7 @@ struct sddc_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t own = nullptr;
-  std::vector<hipStream_t> side;   // per-class capture streams (multi-class graphs)
+  std::vector<hipStream_t> side;   // the capture's lane streams (at most one per hardware queue)
   std::vector<hipEvent_t> events;
 };
 
