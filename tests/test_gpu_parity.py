@@ -324,6 +324,39 @@ def test_spectral_instances_give_the_same_bits(precision):
         assert np.array_equal(o, outs[0]), sp
 
 
+@pytest.mark.parametrize("width", [3, 5, 6, 7])
+@pytest.mark.parametrize("seg", [8, 256])
+def test_mixed_width_launch_covers_every_instance(width, seg):
+    """The mixed-width solves (widths 3, 5-10) launch a one-dimensional grid
+    of their live (field, chain, segment, instance group) items, cut into
+    eight runs by XCD and padded to a multiple of eight.  For batches whose
+    item counts are not multiples of eight, one or four segments, every
+    instance of every field solves to the same bits as that instance alone,
+    and instance 0 to the oracle within 1e-14."""
+    K, Nr, d, dt, Pr, Tau = 64, 20, 0.35, 5e-3, 1.0, 1 / 15.0
+    o = O.build_operators(K, Nr, d, dt, Pr, Tau)
+    op = ops.build(K, Nr, d, dt, Pr, Tau)
+    nr = Nr - 1
+    rng = np.random.default_rng(100 * width + seg)
+    tuning = {"chain_segment": seg, "solve_instances": width}
+    alone = dev.Plan(op, max_batch=1, tuning=tuning)
+    for B in (5, 13, 37):
+        p = dev.Plan(op, max_batch=B, tuning=tuning)
+        g = rng.standard_normal((B, K * nr))
+        for field in range(3):
+            f = H(p.bsub(T(g), field))
+            for b in range(B):
+                assert np.array_equal(f[b], H(alone.bsub(T(g[b:b + 1]), field))[0]), (B, field, b)
+            if field == 0:
+                ref = O.a4_bsub(g[0], o.Linv_A4, o.D2_A4, o.IR4_A4, o.IR2_A4, K, nr, Pr * dt, False)
+            else:
+                ref = O.nab2_bsub(g[0], o.Linv_T if field == 1 else o.Linv_S, K, nr,
+                                  dt if field == 1 else Tau * dt, False)
+            assert rel(f[0], ref) < 1e-14, (B, field)
+        p.close()
+    alone.close()
+
+
 @pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 16])
 def test_solve_widths_padded_instances(width):
     """Regression test: batches that do not fill the last workgroup of a
