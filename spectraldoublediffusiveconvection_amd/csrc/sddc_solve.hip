@@ -848,8 +848,9 @@ __device__ __forceinline__ void join_a4(const ChainArgs& a, int chain, int start
 template <int NRS, int NB>
 __global__ __launch_bounds__(64 * CShape<NRS>::WR) void k_chain_join(ChainArgs a) {
   __shared__ __align__(16) double xs[2][NB][2 * NRS + 64];
-  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y;
-  if (seg == 0 || !((a.fields_mask >> field) & 1)) return;
+  // segments 1 .. nseg-1 (segment 0 starts from a zero state: nothing to join)
+  const int field = blockIdx.x >> 1, chain = blockIdx.x & 1, seg = blockIdx.y + 1;
+  if (!((a.fields_mask >> field) & 1)) return;
   if (a.symmetric && (field == 0 ? chain == 1 : chain == 0)) return;
   const int start = a.K - 1 - chain;
   const int n = start / 2 + 1;
@@ -1232,7 +1233,7 @@ static int launch_chain_mfma_t(const ChainArgs& a, int B, hipStream_t s) {
   hipLaunchKernelGGL((k_chain_mfma<NRS>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_mfma")) return rc;
   if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup
-    const dim3 gj(chain_grid_x(a), a.nseg, (B + 1) / 2);
+    const dim3 gj(chain_grid_x(a), a.nseg - 1, (B + 1) / 2);
     hipLaunchKernelGGL((k_chain_join<NRS, 2>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
   }
@@ -1252,7 +1253,7 @@ static int launch_chain_t(const ChainArgs& a, int B, hipStream_t s) {
     // library's default width 4 at 64 instances, N_r = 40: the solve took
     // 111 against 121 µs at N_fm = 256 and 197 against 201 µs at 512.
     constexpr int NBJ = (NB > 2 && NRS >= 40) ? 2 : NB;
-    const dim3 gj(chain_grid_x(a), a.nseg, (B + NBJ - 1) / NBJ);
+    const dim3 gj(chain_grid_x(a), a.nseg - 1, (B + NBJ - 1) / NBJ);
     hipLaunchKernelGGL((k_chain_join<NRS, NBJ>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
   }
@@ -1268,7 +1269,7 @@ static int launch_chain_mixed_t(const ChainArgs& a, int B, hipStream_t s) {
   hipLaunchKernelGGL((k_chain_local_mixed<NRS, NBA, NBN, X2, KD>), grid, block, 0, s, a);
   if (int rc = hip_check(hipGetLastError(), "k_chain_local_mixed")) return rc;
   if (a.nseg > 1) {   // joins in the quad layout, two instances per workgroup
-    const dim3 gj(chain_grid_x(a), a.nseg, (B + 1) / 2);
+    const dim3 gj(chain_grid_x(a), a.nseg - 1, (B + 1) / 2);
     hipLaunchKernelGGL((k_chain_join<NRS, 2>), gj, block, 0, s, a);
     if (int rc = hip_check(hipGetLastError(), "k_chain_join")) return rc;
   }
